@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput of the all-pairs normalized cross-correlation on one MI355X.
+
+Workload (the headline configuration): N=4096 events x 3 channels x 2048
+float32 samples, seeded noise + Ricker events, FFT period P=4096, full lag
+range (-2047..2048), channel coefficients averaged per lag.  One step = one
+pass of the hot path over the resident batch: batched R2C of all 12288
+traces, the fused pair kernel over all N(N+1)/2 pairs (conj-multiply +
+channel stack + C2R + max/argmax, min/argmin), and the FP64 near-tie
+resolver.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (stdout).  value = unique pairs N(N-1)/2 per second.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "pairs/sec N=4096 L=2048: FP32-FFT vs MFMA-bf16x3; serve p50/p99 ms"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fft_flops_per_pair(C: int, P: int) -> float:
+    """Algorithmic work of the FFT form per pair: C conj-multiply-accumulates
+    (8 flops per bin) plus one stacked C2R (2.5 P log2 P)."""
+    B = P // 2 + 1
+    return C * 8.0 * B + 2.5 * P * math.log2(P)
+
+
+def cpu_baseline(ev: np.ndarray, P: int, S: int, sample_events: int, threads: int):
+    from oracle import oracle
+    n = min(sample_events, ev.shape[0])
+    C, L = ev.shape[1], ev.shape[2]
+    padded = np.zeros((n, C, P), np.float32)
+    padded[:, :, :L] = ev[:n]
+    oracle.load()
+    t0 = time.perf_counter()
+    oracle.allpairs_stacked(padded, L, S, num_threads=threads, chunk_size=4)
+    dt = time.perf_counter() - t0
+    pairs = n * (n - 1) / 2
+    return {"value": pairs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"all {n * (n + 1) // 2} packed pairs of the first {n} events "
+                      f"(C={C}, L={L}, P={P}, full lags), {dt:.2f} s wall, C/OpenMP "
+                      f"restatement of Correlation_in_OMP with channel stacking, own FP32 FFT"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-events", type=int, default=4096)
+    ap.add_argument("--cpu-sample-events", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (scales PMC byte counters)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one MI355X; --gpus {args.gpus} (WORLD_SIZE={world}) is not "
+              "supported: the all-pairs job runs on a single GPU", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from seismic_amd import engine, synth
+
+    N, C, L, P = args.n_events, 3, 2048, 4096
+    lag_lo, lag_hi = -P // 2 + 1, P // 2
+    log(f"bench: generating N={N} C={C} L={L}")
+    ev, _ = synth.make_events(N, C, L, seed=20240601)
+    dev = torch.device("cuda:0")
+    tr = torch.from_numpy(ev).to(dev)
+    plan = engine.AllPairsPlan(N, C, L, P, lag_lo, lag_hi, threshold=0.0)
+    outs = plan.alloc_outputs(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    if args.copy_calibration:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).uniform_()
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("bench: copy calibration 3 x 256 MiB done")
+
+    for _ in range(args.warmup):
+        plan.execute(tr, out=outs, stream=stream)
+    torch.cuda.synchronize()
+    st = plan.stats()
+    log(f"bench: warmup done, near ties per step {st['near_ties']}")
+
+    plan.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute(tr, out=outs, stream=stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tm = plan.timing()
+    st = plan.stats()
+    ms = (t1 - t0) * 1e3 / args.steps
+    unique = N * (N - 1) / 2
+    packed = N * (N + 1) / 2
+    value = unique / (ms / 1e3)
+    log(f"bench: {ms:.3f} ms/step, pair kernel {tm['ms_pairs']:.3f} ms, forward "
+        f"{tm['ms_forward']:.3f} ms, resolve {tm['ms_resolve']:.3f} ms")
+
+    fpp = fft_flops_per_pair(C, P)
+    achieved = packed * fpp / (tm["ms_pairs"] / 1e3) / 1e12
+    traffic = None
+    prof = os.path.join(HERE, "profiles", "pairs_fft_n4096_c3_p4096_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": traffic, "kernel": "k_pairs<32,full>",
+                "compute": "FP32 vector ALU (no MFMA); peak is MI355X FP32 157.3 TFLOP/s",
+                "flops_per_pair": fpp, "pairs_per_launch": int(packed),
+                "kernel_ms": round(tm["ms_pairs"], 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"bench: CPU baseline on {threads} threads, {args.cpu_sample_events} events")
+        cpu = cpu_baseline(ev, P, P // 2, args.cpu_sample_events, threads)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded band-passed noise + 5 Hz Ricker, SNR U[0.5,10])",
+        "config": {"workload": "all-pairs NCC, FP32-FFT form", "form": "fft_f32",
+                   "n_events": N, "channels": C, "samples": L, "padded_size": P,
+                   "lags": [lag_lo, lag_hi], "unique_pairs": int(unique),
+                   "packed_pairs_computed": int(packed)},
+        "stages_ms": {k: round(v, 4) for k, v in tm.items() if k != "executes"},
+        "near_ties_per_step": st["near_ties"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "mfma_bf16x3": None,
+        "serving": None,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

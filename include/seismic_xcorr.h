@@ -1,0 +1,105 @@
+/*
+ * seismic_xcorr.h -- extended C ABI of libseismic_xcorr.so, the MI355X
+ * (gfx950) engine behind the drop-in libraries correlation_cuda.so and
+ * correlation_multicuda.so.
+ *
+ * The drop-in entry points keep the reference's exact signatures and are
+ * declared in correlation_cuda.h and correlation_multicuda.h.  Everything
+ * here is additional: device-resident plans for callers that keep traces in
+ * HBM, the error string the void drop-in functions cannot return, and the
+ * lag-window (MFMA) and serving forms.
+ *
+ * Conventions shared by every entry point
+ *   - lag l > 0 means the row event i is delayed relative to the column event
+ *     j: the coefficient at lag l is sum_n x_i[n] x_j[n - l] / (|x_i| |x_j|),
+ *     averaged over channels;
+ *   - packed output index of pair (i, j), j >= i: i*N - i*(i-1)/2 + (j - i)
+ *     (Correlation_in_OMP/correlation_c.cpp:269), 64-bit;
+ *   - "pos" is the largest coefficient above 0 and its first lag, "neg" the
+ *     smallest below 0 (Correlation_in_OMP/correlation_c.cpp:102-111); a
+ *     value not beyond the threshold is stored as (0, 0) (:119-136);
+ *   - near ties (another lag within tie_tol of the winner) are settled by an
+ *     FP64 re-evaluation of the candidate lags, ties to the smaller lag;
+ *   - return value 0 on success, negative on error (see seismic_last_error).
+ */
+#ifndef SEISMIC_XCORR_H
+#define SEISMIC_XCORR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SXC_OK = 0,
+  SXC_ERR_INVALID = -1,   /* bad argument or unsupported shape */
+  SXC_ERR_HIP = -2,       /* a HIP runtime call failed */
+  SXC_ERR_MEMORY = -3,    /* device allocation failed */
+};
+
+enum {
+  SXC_FORM_FFT_F32 = 0,     /* batched R2C + fused conj-multiply/C2R/argmax, FP32 */
+};
+
+typedef struct sxc_plan sxc_plan;
+
+typedef struct {
+  int n_events;      /* N >= 1 */
+  int n_channels;    /* C in [1, 8]; channel correlations are averaged per lag */
+  int trace_len;     /* L: samples entering the energy (and the mean) */
+  int padded_size;   /* P: FFT period, a power of two in [1024, 8192]; lags are circular mod P */
+  int lag_lo;        /* first scanned lag, > -P/2 */
+  int lag_hi;        /* last scanned lag, <= P/2, >= lag_lo */
+  float threshold;   /* pos kept if > threshold, neg if < -threshold */
+  float tie_tol;     /* near-tie band in coefficient units; <= 0 selects 1e-5 */
+  int demean;        /* 1: subtract the FP64 mean of the first L samples of each trace */
+  int form;          /* SXC_FORM_* */
+} sxc_plan_desc;
+
+typedef struct {
+  unsigned long long pairs;       /* pairs computed by the last execute (incl. diagonal) */
+  unsigned long long near_ties;   /* pairs re-evaluated in FP64 by the last execute */
+  unsigned long long tie_overflow;/* near ties beyond the record capacity (0 expected) */
+} sxc_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char* seismic_last_error(void);
+
+/* Allocates the plan's device workspace (spectra, norms, work list, twiddles). */
+int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out);
+int sxc_plan_destroy(sxc_plan* plan);
+
+/* Runs the all-pairs job on device-resident traces.
+ *   d_traces: [n_events][n_channels][trace_stride] float32 on the device;
+ *             samples at index >= in_len are taken as zero (in_len <= P).
+ *   outputs:  device arrays of n_events*(n_events+1)/2 entries.
+ *   stream:   hipStream_t (NULL = default stream); the call is asynchronous. */
+int sxc_plan_execute(sxc_plan* plan, const float* d_traces, long long trace_stride, int in_len,
+                     float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                     void* stream);
+
+/* Counters of the last execute on this plan (synchronises its stream). */
+int sxc_plan_stats(sxc_plan* plan, sxc_stats* out);
+
+/* Per-stage device time of the executes since timing was enabled, measured
+ * with HIP events recorded on the execute stream around each launch. */
+typedef struct {
+  int executes;          /* executes measured */
+  double ms_forward;     /* average per execute: batched R2C of every trace */
+  double ms_pairs;       /* average per execute: fused pair kernel */
+  double ms_resolve;     /* average per execute: FP64 near-tie resolver */
+} sxc_timing;
+
+/* on != 0 starts (and resets) per-stage timing for up to 1024 executes. */
+int sxc_plan_enable_timing(sxc_plan* plan, int on);
+/* Synchronises the recorded events and returns the averages. */
+int sxc_plan_timing(sxc_plan* plan, sxc_timing* out);
+
+/* Host-memory convenience wrapper: copies traces in, runs, copies the packed
+ * outputs back.  traces: [n_events][n_channels][trace_stride] float32. */
+int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,
+                      int in_len, float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEISMIC_XCORR_H */

@@ -1,0 +1,98 @@
+"""ctypes loading of the in-tree HIP libraries (no fallback: a missing or
+unloadable library raises)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class SxcPlanDesc(ctypes.Structure):
+    """Mirror of sxc_plan_desc (include/seismic_xcorr.h)."""
+    _fields_ = [
+        ("n_events", ctypes.c_int),
+        ("n_channels", ctypes.c_int),
+        ("trace_len", ctypes.c_int),
+        ("padded_size", ctypes.c_int),
+        ("lag_lo", ctypes.c_int),
+        ("lag_hi", ctypes.c_int),
+        ("threshold", ctypes.c_float),
+        ("tie_tol", ctypes.c_float),
+        ("demean", ctypes.c_int),
+        ("form", ctypes.c_int),
+    ]
+
+
+class SxcStats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_ulonglong),
+        ("near_ties", ctypes.c_ulonglong),
+        ("tie_overflow", ctypes.c_ulonglong),
+    ]
+
+
+class SxcTiming(ctypes.Structure):
+    _fields_ = [
+        ("executes", ctypes.c_int),
+        ("ms_forward", ctypes.c_double),
+        ("ms_pairs", ctypes.c_double),
+        ("ms_resolve", ctypes.c_double),
+    ]
+
+
+# every symbol include/*.h declares, per library
+EXPORTS = {
+    "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
+                            "sxc_plan_execute", "sxc_plan_stats", "sxc_allpairs_host",
+                            "sxc_plan_enable_timing", "sxc_plan_timing"],
+    "correlation_cuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
+    "correlation_multicuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
+}
+
+
+def library_path(name: str) -> str:
+    return os.path.join(LIB_DIR, name)
+
+
+_engine = None
+
+
+def load_engine() -> ctypes.CDLL:
+    """Load libseismic_xcorr.so and declare its argtypes."""
+    global _engine
+    if _engine is not None:
+        return _engine
+    path = library_path("libseismic_xcorr.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m seismic_amd.build`")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.seismic_last_error.restype = ctypes.c_char_p
+    lib.seismic_last_error.argtypes = []
+    lib.sxc_plan_create.restype = ctypes.c_int
+    lib.sxc_plan_create.argtypes = [ctypes.POINTER(SxcPlanDesc), ctypes.POINTER(vp)]
+    lib.sxc_plan_destroy.restype = ctypes.c_int
+    lib.sxc_plan_destroy.argtypes = [vp]
+    lib.sxc_plan_execute.restype = ctypes.c_int
+    lib.sxc_plan_execute.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, vp, vp]
+    lib.sxc_plan_stats.restype = ctypes.c_int
+    lib.sxc_plan_stats.argtypes = [vp, ctypes.POINTER(SxcStats)]
+    lib.sxc_plan_enable_timing.restype = ctypes.c_int
+    lib.sxc_plan_enable_timing.argtypes = [vp, ctypes.c_int]
+    lib.sxc_plan_timing.restype = ctypes.c_int
+    lib.sxc_plan_timing.argtypes = [vp, ctypes.POINTER(SxcTiming)]
+    lib.sxc_allpairs_host.restype = ctypes.c_int
+    lib.sxc_allpairs_host.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, ctypes.c_longlong,
+                                      ctypes.c_int, c_float_p, c_int_p, c_float_p, c_int_p]
+    _engine = lib
+    return lib
+
+
+def last_error(lib=None) -> str:
+    lib = lib or load_engine()
+    s = lib.seismic_last_error()
+    return s.decode() if s else ""

@@ -1,0 +1,506 @@
+// Host engine of libseismic_xcorr.so: plans, device workspace, launch order
+// and the extended C ABI (include/seismic_xcorr.h).
+//
+// Orchestration it replaces: correlationCUDA, Correlation_in_CUDA/
+// correlation_cuda.cu:530-673 (allocate, copy in, norms, forward FFTs,
+// chunked multiply/C2R/feature-extraction loop :409-505, copy out), with
+// every HIP call checked instead of the reference's unchecked calls.
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/seismic_xcorr.h"
+#include "xc_engine.h"
+#include "xc_kernels.h"
+
+namespace sxc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  if (getenv("SEISMIC_XCORR_VERBOSE")) fprintf(stderr, "seismic_xcorr: %s\n", buf);
+}
+void clear_error() { g_last_error.clear(); }
+const char* last_error() { return g_last_error.c_str(); }
+
+#define HIP_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+      return SXC_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+// exp(sign * 2 pi i k / n) in FP64 with the angle reduced to an octant, then
+// rounded once to FP32 (same construction as the device's compile-time table).
+static void twiddle(long k, long n, int sign, float* re, float* im) {
+  k %= n;
+  if (k < 0) k += n;
+  long k8 = 8 * k, oct = k8 / n, rem = k8 - oct * n;
+  double a = (double)rem / (double)n * (M_PI / 4.0);
+  double ca = cos(a), sa = sin(a), cb = cos(M_PI / 4.0 - a), sb = sin(M_PI / 4.0 - a);
+  if (rem == 0) { ca = 1.0; sa = 0.0; cb = sqrt(0.5); sb = sqrt(0.5); }
+  double c, s;
+  switch (oct) {
+    case 0: c = ca; s = sa; break;
+    case 1: c = sb; s = cb; break;
+    case 2: c = -sa; s = ca; break;
+    case 3: c = -cb; s = sb; break;
+    case 4: c = -ca; s = -sa; break;
+    case 5: c = -sb; s = -cb; break;
+    case 6: c = sa; s = -ca; break;
+    default: c = cb; s = -sb; break;
+  }
+  *re = (float)c;
+  *im = (float)(sign * s);
+}
+
+template <typename T>
+static int dev_alloc(T** p, size_t count, const char* what) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    set_error("device allocation of %s (%zu bytes) failed: %s", what, count * sizeof(T), hipGetErrorString(e));
+    *p = nullptr;
+    return SXC_ERR_MEMORY;
+  }
+  return SXC_OK;
+}
+
+int validate_desc(const sxc_plan_desc& d) {
+  const int P = d.padded_size;
+  if (d.n_events < 1) { set_error("n_events must be >= 1 (got %d)", d.n_events); return SXC_ERR_INVALID; }
+  if (d.n_channels < 1 || d.n_channels > 8) { set_error("n_channels must be in [1, 8] (got %d)", d.n_channels); return SXC_ERR_INVALID; }
+  if (P < 1024 || P > 8192 || (P & (P - 1))) {
+    set_error("padded_size must be a power of two in [1024, 8192] (got %d)", P);
+    return SXC_ERR_INVALID;
+  }
+  if (d.trace_len < 1 || d.trace_len > P) { set_error("trace_len must be in [1, padded_size] (got %d)", d.trace_len); return SXC_ERR_INVALID; }
+  if (d.lag_lo > d.lag_hi || d.lag_lo <= -P / 2 || d.lag_hi > P / 2) {
+    set_error("lag window [%d, %d] must satisfy -P/2 < lo <= hi <= P/2 (P=%d)", d.lag_lo, d.lag_hi, P);
+    return SXC_ERR_INVALID;
+  }
+  if (d.form != SXC_FORM_FFT_F32) { set_error("unsupported form %d", d.form); return SXC_ERR_INVALID; }
+  if (!(d.threshold == d.threshold)) { set_error("threshold is NaN"); return SXC_ERR_INVALID; }
+  long long pairs = (long long)d.n_events * (d.n_events + 1) / 2;
+  if (pairs > (1LL << 40)) { set_error("too many pairs"); return SXC_ERR_INVALID; }
+  return SXC_OK;
+}
+
+}  // namespace sxc
+
+using namespace sxc;
+
+struct sxc_plan {
+  sxc_plan_desc d;
+  int R = 0, M = 0, P = 0;
+  bool full = false;
+  long long spec_stride = 0;  // float2 per (event, channel) row
+  long long n_pairs = 0;
+  int n_work = 0, chunk = 32, tie_cap = 0;
+  float tie_tol = 1e-5f;
+  // device workspace
+  float2* spec = nullptr;
+  double* norms = nullptr;
+  float* traces = nullptr;     // demeaned copy for the FP64 resolver (demean only)
+  int2* work = nullptr;
+  int4* ties = nullptr;
+  unsigned* tie_count = nullptr;
+  float2 *fwd2 = nullptr, *fwd64 = nullptr, *inv2 = nullptr, *inv64 = nullptr, *post = nullptr, *pre = nullptr;
+  hipStream_t last_stream = nullptr;
+  int device = 0;
+  // stage timing: 4 events per execute (start, after forward, pairs, resolve)
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  int timed = 0;
+};
+
+static void plan_free(sxc_plan* p) {
+  if (!p) return;
+  for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+  void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
+                  p->fwd2, p->fwd64, p->inv2, p->inv64, p->post, p->pre};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete p;
+}
+
+static int upload_tables(sxc_plan* p) {
+  const int R = p->R, M = p->M, P = p->P;
+  std::vector<float2> f2(R * 64), i2(R * 64), f64(64), i64(64), post(M + 1), pre(64);
+  for (int k = 0; k < R; ++k)
+    for (int t = 0; t < 64; ++t) {
+      twiddle((long)t * k, M, -1, &f2[k * 64 + t].x, &f2[k * 64 + t].y);
+      twiddle((long)t * k, M, +1, &i2[k * 64 + t].x, &i2[k * 64 + t].y);
+    }
+  for (int k = 0; k < 8; ++k)
+    for (int t = 0; t < 8; ++t) {
+      twiddle((long)t * k, 64, -1, &f64[k * 8 + t].x, &f64[k * 8 + t].y);
+      twiddle((long)t * k, 64, +1, &i64[k * 8 + t].x, &i64[k * 8 + t].y);
+    }
+  for (int k = 0; k <= M; ++k) twiddle(k, P, -1, &post[k].x, &post[k].y);
+  for (int t = 0; t < 64; ++t) twiddle(t, P, +1, &pre[t].x, &pre[t].y);
+  int rc;
+  if ((rc = dev_alloc(&p->fwd2, f2.size(), "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->inv2, i2.size(), "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->fwd64, 64, "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->inv64, 64, "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->post, post.size(), "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->pre, 64, "twiddles"))) return rc;
+  HIP_CHECK(hipMemcpy(p->fwd2, f2.data(), f2.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->inv2, i2.data(), i2.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->fwd64, f64.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->inv64, i64.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->post, post.data(), post.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->pre, pre.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
+  return SXC_OK;
+}
+
+// Work items (row i, first column) over column chunks of `chunk` events,
+// ordered chunk-major so the workgroups in flight share one chunk of column
+// spectra in L2.
+static std::vector<int2> build_work(int n, int chunk) {
+  std::vector<int2> w;
+  const int nc = (n + chunk - 1) / chunk;
+  for (int c = 0; c < nc; ++c) {
+    const int j0 = c * chunk, j1 = std::min(n, j0 + chunk);
+    for (int i = 0; i < j1; ++i) w.push_back(make_int2(i, std::max(i, j0)));
+  }
+  return w;
+}
+
+extern "C" {
+
+const char* seismic_last_error(void) { return last_error(); }
+
+int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
+  clear_error();
+  if (!desc || !out) { set_error("null argument"); return SXC_ERR_INVALID; }
+  *out = nullptr;
+  int rc = validate_desc(*desc);
+  if (rc) return rc;
+  sxc_plan* p = new sxc_plan();
+  p->d = *desc;
+  p->P = desc->padded_size;
+  p->M = p->P / 2;
+  p->R = p->M / 64;
+  p->full = (desc->lag_lo == -p->M + 1 && desc->lag_hi == p->M);
+  p->spec_stride = p->M + 32;  // 256-byte aligned rows, bins 0..M
+  p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
+  p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol : 1e-5f;
+  p->tie_cap = (int)std::min<long long>(p->n_pairs, 1LL << 24);
+  HIP_CHECK(hipGetDevice(&p->device));
+  const long long rows = (long long)desc->n_events * desc->n_channels;
+  auto fail = [&](int code) { plan_free(p); return code; };
+  if ((rc = dev_alloc(&p->spec, (size_t)(rows * p->spec_stride), "spectra"))) return fail(rc);
+  if ((rc = dev_alloc(&p->norms, (size_t)rows, "norms"))) return fail(rc);
+  if (desc->demean && (rc = dev_alloc(&p->traces, (size_t)(rows * p->P), "demeaned traces"))) return fail(rc);
+  std::vector<int2> work = build_work(desc->n_events, p->chunk);
+  p->n_work = (int)work.size();
+  if ((rc = dev_alloc(&p->work, work.size(), "work list"))) return fail(rc);
+  if (hipMemcpy(p->work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("copying the work list failed");
+    return fail(SXC_ERR_HIP);
+  }
+  if ((rc = dev_alloc(&p->ties, (size_t)p->tie_cap, "near-tie records"))) return fail(rc);
+  if ((rc = dev_alloc(&p->tie_count, 1, "near-tie counter"))) return fail(rc);
+  if ((rc = upload_tables(p))) return fail(rc);
+  *out = p;
+  return SXC_OK;
+}
+
+int sxc_plan_destroy(sxc_plan* plan) {
+  plan_free(plan);
+  return SXC_OK;
+}
+
+int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                     float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                     void* stream) {
+  clear_error();
+  if (!p || !d_traces || !d_vals_pos || !d_lags_pos || !d_vals_neg || !d_lags_neg) {
+    set_error("null argument");
+    return SXC_ERR_INVALID;
+  }
+  if (in_len < 1 || in_len > p->P || trace_stride < in_len) {
+    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  p->last_stream = st;
+  const sxc_plan_desc& d = p->d;
+  const int rows = d.n_events * d.n_channels;
+
+  ForwardArgs fa{};
+  fa.in = d_traces;
+  fa.in_stride = trace_stride;
+  fa.in_len = in_len;
+  fa.L = d.trace_len;
+  fa.n_rows = rows;
+  fa.n_channels = d.n_channels;
+  fa.demean = d.demean;
+  fa.norms = p->norms;
+  fa.out_traces = p->traces;
+  fa.out_stride = p->P;
+  fa.spec = p->spec;
+  fa.spec_stride = p->spec_stride;
+  fa.tab2 = p->fwd2;
+  fa.tab64 = p->fwd64;
+  fa.post_tw = p->post;
+  hipEvent_t* ev = nullptr;
+  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
+  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
+  HIP_CHECK(launch_forward(p->R, fa, st));
+  HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
+
+  PairArgs pa{};
+  pa.spec = p->spec;
+  pa.spec_stride = p->spec_stride;
+  pa.n_events = d.n_events;
+  pa.n_channels = d.n_channels;
+  pa.work = p->work;
+  pa.n_work = p->n_work;
+  pa.chunk = p->chunk;
+  pa.lag_lo = d.lag_lo;
+  pa.lag_hi = d.lag_hi;
+  pa.threshold = d.threshold;
+  pa.tie_tol = p->tie_tol;
+  pa.tab2 = p->inv2;
+  pa.tab64 = p->inv64;
+  pa.pre_lane = p->pre;
+  pa.vpos = d_vals_pos;
+  pa.lpos = d_lags_pos;
+  pa.vneg = d_vals_neg;
+  pa.lneg = d_lags_neg;
+  pa.ties = p->ties;
+  pa.tie_count = p->tie_count;
+  pa.tie_cap = p->tie_cap;
+  const int grid = std::min(p->n_work, 8192);
+  HIP_CHECK(launch_pairs(p->R, p->full, pa, grid, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
+
+  ResolveArgs ra{};
+  ra.spec = p->spec;
+  ra.spec_stride = p->spec_stride;
+  ra.n_events = d.n_events;
+  ra.n_channels = d.n_channels;
+  ra.norms = p->norms;
+  if (p->traces) {
+    ra.traces = p->traces;
+    ra.tr_stride = p->P;
+    ra.tr_len = p->P;
+  } else {
+    ra.traces = d_traces;
+    ra.tr_stride = trace_stride;
+    ra.tr_len = in_len;
+  }
+  ra.lag_lo = d.lag_lo;
+  ra.lag_hi = d.lag_hi;
+  ra.threshold = d.threshold;
+  ra.tie_tol = p->tie_tol;
+  ra.tab2 = p->inv2;
+  ra.tab64 = p->inv64;
+  ra.pre_lane = p->pre;
+  ra.vpos = d_vals_pos;
+  ra.lpos = d_lags_pos;
+  ra.vneg = d_vals_neg;
+  ra.lneg = d_lags_neg;
+  ra.ties = p->ties;
+  ra.tie_count = p->tie_count;
+  ra.tie_cap = p->tie_cap;
+  HIP_CHECK(launch_resolve(p->R, ra, 512, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
+  return SXC_OK;
+}
+
+int sxc_plan_enable_timing(sxc_plan* p, int on) {
+  clear_error();
+  if (!p) { set_error("null plan"); return SXC_ERR_INVALID; }
+  p->timing = on != 0;
+  p->timed = 0;
+  if (p->timing && p->events.empty()) {
+    p->events.resize(4 * 1024);
+    for (auto& e : p->events) HIP_CHECK(hipEventCreate(&e));
+  }
+  return SXC_OK;
+}
+
+int sxc_plan_timing(sxc_plan* p, sxc_timing* out) {
+  clear_error();
+  if (!p || !out) { set_error("null argument"); return SXC_ERR_INVALID; }
+  memset(out, 0, sizeof(*out));
+  for (int k = 0; k < p->timed; ++k) {
+    hipEvent_t* e = &p->events[4 * k];
+    HIP_CHECK(hipEventSynchronize(e[3]));
+    float a = 0, b = 0, c = 0;
+    HIP_CHECK(hipEventElapsedTime(&a, e[0], e[1]));
+    HIP_CHECK(hipEventElapsedTime(&b, e[1], e[2]));
+    HIP_CHECK(hipEventElapsedTime(&c, e[2], e[3]));
+    out->ms_forward += a;
+    out->ms_pairs += b;
+    out->ms_resolve += c;
+  }
+  out->executes = p->timed;
+  if (p->timed) {
+    out->ms_forward /= p->timed;
+    out->ms_pairs /= p->timed;
+    out->ms_resolve /= p->timed;
+  }
+  return SXC_OK;
+}
+
+int sxc_plan_stats(sxc_plan* p, sxc_stats* out) {
+  clear_error();
+  if (!p || !out) { set_error("null argument"); return SXC_ERR_INVALID; }
+  HIP_CHECK(hipStreamSynchronize(p->last_stream));
+  unsigned cnt = 0;
+  HIP_CHECK(hipMemcpy(&cnt, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToHost));
+  out->pairs = (unsigned long long)p->n_pairs;
+  out->near_ties = std::min<unsigned long long>(cnt, (unsigned long long)p->tie_cap);
+  out->tie_overflow = cnt > (unsigned)p->tie_cap ? cnt - (unsigned)p->tie_cap : 0;
+  return SXC_OK;
+}
+
+int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,
+                      int in_len, float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg) {
+  clear_error();
+  if (!desc || !traces || !vals_pos || !lags_pos || !vals_neg || !lags_neg) {
+    set_error("null argument");
+    return SXC_ERR_INVALID;
+  }
+  sxc_plan* p = nullptr;
+  int rc = sxc_plan_create(desc, &p);
+  if (rc) return rc;
+  const long long rows = (long long)desc->n_events * desc->n_channels;
+  const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
+  const size_t np = (size_t)p->n_pairs;
+  float *d_in = nullptr, *d_vp = nullptr, *d_vn = nullptr;
+  int *d_lp = nullptr, *d_ln = nullptr;
+  std::string err;
+  auto cleanup = [&]() {
+    void* bufs[] = {d_in, d_vp, d_vn, d_lp, d_ln};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    sxc_plan_destroy(p);
+  };
+  if ((rc = dev_alloc(&d_in, (size_t)(rows * trace_stride), "input traces")) ||
+      (rc = dev_alloc(&d_vp, np, "output values")) || (rc = dev_alloc(&d_vn, np, "output values")) ||
+      (rc = dev_alloc(&d_lp, np, "output lags")) || (rc = dev_alloc(&d_ln, np, "output lags"))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_in, traces, in_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("copying traces to the device failed: %s", hipGetErrorString(e));
+    cleanup();
+    return SXC_ERR_HIP;
+  }
+  rc = sxc_plan_execute(p, d_in, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr);
+  if (rc == SXC_OK) {
+    hipError_t es = hipDeviceSynchronize();
+    if (es != hipSuccess) {
+      set_error("device execution failed: %s", hipGetErrorString(es));
+      rc = SXC_ERR_HIP;
+    }
+  }
+  if (rc == SXC_OK) {
+    hipError_t e1 = hipMemcpy(vals_pos, d_vp, np * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(lags_pos, d_lp, np * sizeof(int), hipMemcpyDeviceToHost);
+    hipError_t e3 = hipMemcpy(vals_neg, d_vn, np * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t e4 = hipMemcpy(lags_neg, d_ln, np * sizeof(int), hipMemcpyDeviceToHost);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+      set_error("copying results to the host failed");
+      rc = SXC_ERR_HIP;
+    }
+  }
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"
+
+namespace sxc {
+
+// The reference ABI's argument rules (correlation_c.cpp:102-103 reads
+// corr_t[L + i], i in [-S, S), so it needs 0 <= L - S and L + S <= P).
+int reference_desc(int n_events, int event_length, int shift, int padded, float threshold,
+                   sxc_plan_desc* d) {
+  if (n_events < 1) { set_error("n_events must be >= 1 (got %d)", n_events); return SXC_ERR_INVALID; }
+  if (shift < 0 || event_length - shift < 0 || event_length + shift > padded) {
+    set_error("shift %d out of range for event_length %d and paddedSize %d "
+              "(needs 0 <= shift, event_length - shift >= 0, event_length + shift <= paddedSize)",
+              shift, event_length, padded);
+    return SXC_ERR_INVALID;
+  }
+  memset(d, 0, sizeof(*d));
+  d->n_events = n_events;
+  d->n_channels = 1;
+  d->trace_len = event_length;
+  d->padded_size = padded;
+  d->lag_lo = -shift + 1;
+  d->lag_hi = shift;
+  d->threshold = threshold;
+  d->tie_tol = 0.f;
+  d->demean = 0;
+  d->form = SXC_FORM_FFT_F32;
+  return SXC_OK;
+}
+
+void reference_call(float* events, int n_events, int event_length, int shift, int padded,
+                    float threshold, float* vp, int* lp, float* vn, int* ln) {
+  clear_error();
+  if (!events || !vp || !lp || !vn || !ln) {
+    set_error("null pointer argument");
+    return;
+  }
+  sxc_plan_desc d;
+  if (reference_desc(n_events, event_length, shift, padded, threshold, &d)) return;
+  if (shift == 0) {  // empty scan: every pair is (0, lag 0)
+    const long long np = (long long)n_events * (n_events + 1) / 2;
+    memset(vp, 0, np * sizeof(float));
+    memset(lp, 0, np * sizeof(int));
+    memset(vn, 0, np * sizeof(float));
+    memset(ln, 0, np * sizeof(int));
+    return;
+  }
+  // the reference FFTs the whole padded row and normalises by the first L samples
+  sxc_allpairs_host(&d, events, padded, padded, vp, lp, vn, ln);
+}
+
+int warm_up() {
+  clear_error();
+  int n = 0;
+  HIP_CHECK(hipGetDeviceCount(&n));
+  if (n < 1) {
+    set_error("no HIP device visible");
+    return SXC_ERR_HIP;
+  }
+  HIP_CHECK(hipFree(nullptr));
+  return SXC_OK;
+}
+
+}  // namespace sxc
+
+extern "C" {
+void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
+                        float threshold, float* vp, int* lp, float* vn, int* ln) {
+  sxc::reference_call(events, n_events, event_length, shift, padded, threshold, vp, lp, vn, ln);
+}
+int sxc_warm_up(void) { return sxc::warm_up(); }
+}

@@ -1,0 +1,260 @@
+// One-wavefront FFT building blocks for CDNA4 (gfx950, wave64).
+//
+// A complex FFT of length M = 64*R is held by one wavefront, R values per
+// lane, and computed as a four-step decomposition M = R x 64:
+//   1. lane t holds x[t + 64*m], m < R; a length-R DFT over m runs in
+//      registers (radix-2 DIT, compile-time twiddles, trivial ones skipped);
+//   2. twiddle by W_M^(t*k2);
+//   3. the remaining 64-point DFTs (one per k2) run across lanes as 8 x 8:
+//      two LDS transposes, each followed by a register radix-8 DFT, with a
+//      W_64 twiddle in between.
+// Output: lane l = h + 8g, register d[8s + k] holds X[(g + 8s) + R*(h + 8k)].
+// The transposes are done in R/8 chunks of 8 registers so a wave needs only
+// 576 float2 (4.5 KiB) of LDS whatever R is; strides 72 and 9 make every
+// 16-lane ds_read_b64/ds_write_b64 group bank-conflict free.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace sxc {
+
+struct cf {
+  float x, y;
+};
+__device__ __forceinline__ cf cmk(float x, float y) { return cf{x, y}; }
+__device__ __forceinline__ cf cadd(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf csub(cf a, cf b) { return cf{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+  return cf{__builtin_fmaf(a.x, b.x, -a.y * b.y), __builtin_fmaf(a.x, b.y, a.y * b.x)};
+}
+// a * conj(b)
+__device__ __forceinline__ cf cmulc(cf a, cf b) {
+  return cf{__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -a.x * b.y)};
+}
+__device__ __forceinline__ cf f2c(float2 v) { return cf{v.x, v.y}; }
+__device__ __forceinline__ float2 c2f(cf v) { return make_float2(v.x, v.y); }
+
+// ---------------------------------------------------------------- compile time
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
+  }
+}
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// Taylor series on [0, pi/4]; 12 terms leave < 1e-18 truncation error.
+constexpr double cx_sin_small(double a) {
+  double a2 = a * a, term = a, sum = a;
+  for (int n = 1; n <= 12; ++n) {
+    term = -term * a2 / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cx_cos_small(double a) {
+  double a2 = a * a, term = 1.0, sum = 1.0;
+  for (int n = 1; n <= 12; ++n) {
+    term = -term * a2 / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+// exp(sign * 2 pi i k / n), angle reduced to an octant so multiples of pi/2
+// are exact.
+struct cxd {
+  double re, im;
+};
+constexpr cxd cx_twiddle(long k, long n, int sign) {
+  k %= n;
+  if (k < 0) k += n;
+  long k8 = 8 * k;
+  long oct = k8 / n;
+  long rem = k8 - oct * n;
+  double a = (double)rem / (double)n * (kPi / 4.0);
+  double ca = cx_cos_small(a), sa = cx_sin_small(a);
+  double cb = cx_cos_small(kPi / 4.0 - a), sb = cx_sin_small(kPi / 4.0 - a);
+  if (rem == 0) {
+    ca = 1.0;
+    sa = 0.0;
+  }
+  double c = 0, s = 0;
+  switch (oct) {
+    case 0: c = ca; s = sa; break;
+    case 1: c = sb; s = cb; break;
+    case 2: c = -sa; s = ca; break;
+    case 3: c = -cb; s = sb; break;
+    case 4: c = -ca; s = -sa; break;
+    case 5: c = -sb; s = -cb; break;
+    case 6: c = sa; s = -ca; break;
+    default: c = cb; s = -sb; break;
+  }
+  return cxd{c, sign * s};
+}
+template <long K, long N, int SIGN>
+struct TwC {
+  static constexpr cxd v = cx_twiddle(K, N, SIGN);
+  static constexpr float c = (float)v.re;
+  static constexpr float s = (float)v.im;
+};
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+constexpr int bitrev(int v, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b)
+    if (v & (1 << b)) r |= 1 << (bits - 1 - b);
+  return r;
+}
+
+// b * W_len^k, W = exp(SIGN*2*pi*i/len), with the trivial cases specialised.
+template <int K, int LEN, int SIGN>
+__device__ __forceinline__ cf twmul(cf b) {
+  constexpr int k = K % LEN;
+  if constexpr (k == 0) {
+    return b;
+  } else if constexpr (4 * k == LEN) {  // SIGN * i
+    return SIGN > 0 ? cf{-b.y, b.x} : cf{b.y, -b.x};
+  } else if constexpr (4 * k == 2 * LEN) {  // -1
+    return cf{-b.x, -b.y};
+  } else if constexpr (4 * k == 3 * LEN) {  // -SIGN * i
+    return SIGN > 0 ? cf{b.y, -b.x} : cf{-b.y, b.x};
+  } else if constexpr (8 * k == LEN || 8 * k == 3 * LEN || 8 * k == 5 * LEN || 8 * k == 7 * LEN) {
+    constexpr float h = (float)0.70710678118654752440;
+    constexpr float c = TwC<k, LEN, SIGN>::c, s = TwC<k, LEN, SIGN>::s;
+    // c, s = +-h : (x*c - y*s, x*s + y*c) = h*(+-x -+ y, ...)
+    constexpr float sc = c > 0 ? 1.f : -1.f, ss = s > 0 ? 1.f : -1.f;
+    return cf{h * (sc * b.x - ss * b.y), h * (ss * b.x + sc * b.y)};
+  } else {
+    constexpr float c = TwC<k, LEN, SIGN>::c, s = TwC<k, LEN, SIGN>::s;
+    return cf{__builtin_fmaf(b.x, c, -b.y * s), __builtin_fmaf(b.x, s, b.y * c)};
+  }
+}
+
+// In-place length-N DFT of a register array, natural order in and out.
+template <int N, int SIGN>
+__device__ __forceinline__ void reg_fft(cf (&a)[N]) {
+  constexpr int LG = ilog2c(N);
+  cf b[N];
+  sfor<0, N>([&](auto i) {
+    constexpr int r = bitrev(decltype(i)::value, LG);
+    b[r] = a[decltype(i)::value];
+  });
+  sfor<1, LG + 1>([&](auto st) {
+    constexpr int len = 1 << decltype(st)::value;
+    constexpr int half = len / 2;
+    sfor<0, N / len>([&](auto blk) {
+      sfor<0, half>([&](auto kk) {
+        constexpr int k = decltype(kk)::value;
+        constexpr int ia = decltype(blk)::value * len + k;
+        constexpr int ib = ia + half;
+        cf t = twmul<k, len, SIGN>(b[ib]);
+        b[ib] = csub(b[ia], t);
+        b[ia] = cadd(b[ia], t);
+      });
+    });
+  });
+  sfor<0, N>([&](auto i) { a[decltype(i)::value] = b[decltype(i)::value]; });
+}
+
+// Keeps the compiler from moving LDS accesses across this point and makes
+// this wave's earlier LDS writes visible to its later LDS reads.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kWaveLds = 576;  // float2 per wave for the chunked transposes
+
+// Per-lane twiddles a wave FFT needs, loaded once per kernel.
+//   step 2: W_M^(t*k2), k2 < R.  For R <= 16 all R values sit in registers;
+//           for larger R they are factored as A[k2 % 8] * B[k2 / 8] with
+//           A[a] = W_M^(t*a), B[b] = W_M^(8*t*b) to save registers.
+//   W_64^((t%8)*k), k < 8, between the two radix-8 passes.
+template <int R>
+struct WaveTw {
+  static constexpr bool kFactored = R > 16;
+  static constexpr int kNA = kFactored ? 8 : R;
+  static constexpr int kNB = kFactored ? R / 8 : 1;
+  cf a[kNA];
+  cf b[kNB];
+  cf tw64[8];
+  __device__ __forceinline__ cf step2(int k) const {
+    if constexpr (kFactored) {
+      return (k >> 3) == 0 ? a[k & 7] : ((k & 7) == 0 ? b[k >> 3] : cmul(a[k & 7], b[k >> 3]));
+    } else {
+      return a[k];
+    }
+  }
+};
+
+// Device tables written by the host (FP64 generated, rounded once):
+//   tab2[k2*64 + t] = W_M^(t*k2) for k2 < R   (rows 8b give the B factors)
+//   tab64[k*8 + t1] = W_64^(t1*k)
+template <int R>
+__device__ __forceinline__ void load_wave_tw(WaveTw<R>& w, const float2* __restrict__ tab2,
+                                             const float2* __restrict__ tab64, int lane) {
+#pragma unroll
+  for (int k = 0; k < WaveTw<R>::kNA; ++k) w.a[k] = f2c(tab2[k * 64 + lane]);
+  if constexpr (WaveTw<R>::kFactored) {
+#pragma unroll
+    for (int k = 0; k < WaveTw<R>::kNB; ++k) w.b[k] = f2c(tab2[8 * k * 64 + lane]);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w.tw64[k] = f2c(tab64[k * 8 + (lane & 7)]);
+}
+
+// The four-step wave FFT (see file comment).  `lds` is this wave's private
+// kWaveLds-float2 buffer.
+template <int R, int SIGN>
+__device__ __forceinline__ void wave_fft(cf (&d)[R], const WaveTw<R>& w, float2* lds, int lane) {
+  static_assert(R % 8 == 0, "R must be a multiple of 8");
+  reg_fft<R, SIGN>(d);
+#pragma unroll
+  for (int k = 1; k < R; ++k) d[k] = cmul(d[k], w.step2(k));
+
+  const int t1 = lane & 7;
+  const int g = lane >> 3;
+  // transpose 1 + radix-8 over t2, chunk by chunk
+  sfor<0, R / 8>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    wave_lds_fence();
+#pragma unroll
+    for (int gp = 0; gp < 8; ++gp) lds[gp * 72 + lane] = c2f(d[8 * s + gp]);
+    wave_lds_fence();
+    cf e[8];
+#pragma unroll
+    for (int t2 = 0; t2 < 8; ++t2) e[t2] = f2c(lds[g * 72 + t1 + 8 * t2]);
+    reg_fft<8, SIGN>(e);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) e[k] = cmul(e[k], w.tw64[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d[8 * s + k] = e[k];
+  });
+  // transpose 2 + radix-8 over t1
+  const int h = t1;
+  sfor<0, R / 8>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lds[g * 72 + k * 9 + t1] = c2f(d[8 * s + k]);
+    wave_lds_fence();
+    cf e[8];
+#pragma unroll
+    for (int tp = 0; tp < 8; ++tp) e[tp] = f2c(lds[g * 72 + h * 9 + tp]);
+    reg_fft<8, SIGN>(e);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d[8 * s + k] = e[k];
+  });
+  wave_lds_fence();
+}
+
+// Frequency index held by (lane, register 8s+k) after wave_fft.
+template <int R>
+__host__ __device__ constexpr int wave_out_index(int lane, int reg) {
+  return ((lane >> 3) + 8 * (reg >> 3)) + R * ((lane & 7) + 8 * (reg & 7));
+}
+
+}  // namespace sxc

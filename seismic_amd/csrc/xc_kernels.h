@@ -1,0 +1,88 @@
+// Kernel argument blocks and launchers for the FP32-FFT form.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace sxc {
+
+struct ForwardArgs {
+  const float* in;        // [n_rows][in_stride] float32
+  long long in_stride;
+  int in_len;             // samples read per row (zero beyond)
+  int L;                  // samples entering energy / mean
+  int n_rows;             // events * channels
+  int n_channels;         // C in scale = 1/sqrt(E*C*P)
+  int demean;             // subtract the FP64 mean of the first L samples
+  double* norms;          // [n_rows] sqrt(E) in FP64
+  float* out_traces;      // optional [n_rows][out_stride] copy of the transformed samples (P per row)
+  long long out_stride;
+  float2* spec;           // [n_rows][spec_stride] bins 0..M
+  long long spec_stride;
+  const float2* tab2;     // forward four-step twiddles [R][64]
+  const float2* tab64;    // forward W_64 twiddles [8][8]
+  const float2* post_tw;  // exp(-2 pi i k / P), k = 0..M
+};
+
+struct PairArgs {
+  const float2* spec;     // [n_events][n_channels][spec_stride]
+  long long spec_stride;
+  int n_events;
+  int n_channels;
+  const int2* work;       // (row i, first column) per work item
+  int n_work;
+  int chunk;              // columns per work item
+  int lag_lo, lag_hi;     // scanned lags, inclusive
+  float threshold;
+  float tie_tol;
+  const float2* tab2;     // inverse four-step twiddles [R][64]
+  const float2* tab64;    // inverse W_64 twiddles [8][8]
+  const float2* pre_lane; // exp(+2 pi i t / P), t < 64
+  float* vpos;
+  int* lpos;
+  float* vneg;
+  int* lneg;
+  int4* ties;             // (i, j, sides, 0)
+  unsigned* tie_count;
+  int tie_cap;
+};
+
+struct ResolveArgs {
+  const float2* spec;
+  long long spec_stride;
+  int n_events;
+  int n_channels;
+  const double* norms;    // [n_events * n_channels]
+  const float* traces;    // [n_events * n_channels][tr_stride], zero beyond tr_len
+  long long tr_stride;
+  int tr_len;
+  int lag_lo, lag_hi;
+  float threshold;
+  float tie_tol;
+  const float2* tab2;
+  const float2* tab64;
+  const float2* pre_lane;
+  float* vpos;
+  int* lpos;
+  float* vneg;
+  int* lneg;
+  const int4* ties;
+  const unsigned* tie_count;
+  int tie_cap;
+};
+
+// Per-R launchers, one translation unit per R (xc_kernels.hip with -DSXC_R).
+template <int R> hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st);
+template <int R> hipError_t launch_pairs_r(bool full, const PairArgs& a, int grid, hipStream_t st);
+template <int R> hipError_t launch_resolve_r(const ResolveArgs& a, int grid, hipStream_t st);
+
+// Runtime dispatch over R (xc_dispatch.cpp).
+hipError_t launch_forward(int R, const ForwardArgs& a, hipStream_t st);
+hipError_t launch_pairs(int R, bool full, const PairArgs& a, int grid, hipStream_t st);
+hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st);
+hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
+
+inline size_t pair_lds_bytes(int R, int n_channels) {
+  return (size_t)(n_channels * (64 * R + 1) + 4 * 576) * 8;
+}
+
+}  // namespace sxc

@@ -1,0 +1,507 @@
+// HIP kernels for the FP32-FFT form of the all-pairs normalized
+// cross-correlation (gfx950).
+//
+// Reference path these kernels replace, as one fused pipeline:
+//   cuFFT R2C x2          Correlation_in_CUDA/correlation_cuda.cu:615-636
+//   EWM_Linear            Correlation_in_CUDA/correlation_cuda.cu:23-75
+//   cuFFT C2R             Correlation_in_CUDA/correlation_cuda.cu:424-428,450
+//   FE_Linear             Correlation_in_CUDA/correlation_cuda.cu:78-152
+//   scan/normalise rule   Correlation_in_OMP/correlation_c.cpp:102-136
+//
+// Math.  The reference multiplies X_i by R_j = rfft(pad(flip(x_j))), which is
+// exp(-2 pi i (L-1) k / P) * conj(X_j), inverse-transforms and scans
+// corr_t[L + i], i in [-S, S), as lag i + 1.  corr_t[L + i] equals
+// P * c[(i + 1) mod P] with c the period-P circular cross-correlation
+// c[l] = sum_n x_i[n] x_j[n - l], so these kernels form X_i * conj(X_j)
+// directly and read lag l at index l mod P; no reversed copy is needed.
+//
+// Spectra are pre-normalised in the forward kernel: Xhat = X / sqrt(E*C*P)
+// with the energy E summed in FP64 on the device, so the inverse transform of
+// sum_c Xhat_ic conj(Xhat_jc) is directly the channel-averaged normalized
+// coefficient.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xc_fft.hpp"
+#include "xc_kernels.h"
+
+#ifndef SXC_R
+#error "compile once per R with -DSXC_R=8|16|32|64"
+#endif
+
+namespace sxc {
+
+// ----------------------------------------------------------------------------
+// small helpers
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+#if SXC_R == 8
+__global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+#endif
+
+// ----------------------------------------------------------------------------
+// forward: one wave per (event, channel) row
+// ----------------------------------------------------------------------------
+// Reads row r (samples n < in_len, zero beyond), optionally demeans the first
+// L samples in FP64, sums the energy of the first L samples in FP64, and
+// writes Xhat[k] = rfft(x)[k] * scale for k = 0..M, scale = 1/sqrt(E*C*P)
+// (0 for a zero-energy row).
+template <int R>
+__global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
+  constexpr int M = 64 * R;
+  constexpr int P = 2 * M;
+  extern __shared__ float2 smem[];
+  float2* wl = smem;              // wave FFT transposes
+  float2* zl = smem + kWaveLds;   // natural-order Z, M entries
+  const int lane = threadIdx.x;
+  const int row = blockIdx.x;
+  if (row >= a.n_rows) return;
+  WaveTw<R> tw;
+  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+
+  const float* x = a.in + (long long)row * a.in_stride;
+  // mean and energy of the first L samples, FP64, straight from memory
+  double s1 = 0.0;
+  for (int n = lane; n < a.L; n += 64) s1 += (double)(n < a.in_len ? x[n] : 0.f);
+  const double mean = a.demean ? wave_sum_f64(s1) / (double)a.L : 0.0;
+  double s2 = 0.0;
+  for (int n = lane; n < a.L; n += 64) {
+    const float v = n < a.in_len ? x[n] : 0.f;
+    const float u = a.demean ? (float)((double)v - mean) : v;
+    s2 += (double)u * (double)u;
+  }
+  const double E = wave_sum_f64(s2);
+  // d[m] = x[2(t+64m)] + i x[2(t+64m)+1]; clamped loads + selects, no branches
+  cf d[R];
+  const int last = a.in_len > 0 ? a.in_len - 1 : 0;
+#pragma unroll
+  for (int m = 0; m < R; ++m) {
+    const int n = 2 * (lane + 64 * m);
+    const float e0 = x[min(n, last)], e1 = x[min(n + 1, last)];
+    d[m].x = n < a.in_len ? e0 : 0.f;
+    d[m].y = n + 1 < a.in_len ? e1 : 0.f;
+  }
+  if (a.demean) {
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int n = 2 * (lane + 64 * m);
+      const float u0 = (float)((double)d[m].x - mean), u1 = (float)((double)d[m].y - mean);
+      d[m].x = n < a.L ? u0 : d[m].x;
+      d[m].y = n + 1 < a.L ? u1 : d[m].y;
+    }
+  }
+  if (a.out_traces) {
+    float* o = a.out_traces + (long long)row * a.out_stride;
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int n = 2 * (lane + 64 * m);
+      *reinterpret_cast<float2*>(o + n) = make_float2(d[m].x, d[m].y);
+    }
+  }
+  if (lane == 0) a.norms[row] = sqrt(E);
+  const float scale = E > 0.0 ? (float)(1.0 / sqrt(E * (double)a.n_channels * (double)P)) : 0.f;
+
+  wave_fft<R, -1>(d, tw, wl, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) zl[wave_out_index<R>(lane, r)] = c2f(d[r]);
+  wave_lds_fence();
+
+  float2* out = a.spec + (long long)row * a.spec_stride;
+  // X[k] = (Z[k] + conj Z[M-k])/2 - i/2 exp(-2 pi i k/P) (Z[k] - conj Z[M-k])
+  for (int k = lane; k <= M; k += 64) {
+    cf z = f2c(zl[k & (M - 1)]);
+    cf zm = f2c(zl[(M - k) & (M - 1)]);
+    zm.y = -zm.y;
+    cf e = cmk(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
+    cf dd = csub(z, zm);
+    cf od = cmk(0.5f * dd.y, -0.5f * dd.x);  // -i/2 * dd
+    cf t = f2c(a.post_tw[k]);
+    cf X = cadd(e, cmul(od, t));
+    if (k == 0 || k == M) X.y = 0.f;
+    out[k] = make_float2(X.x * scale, X.y * scale);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// per-pair inverse transform (shared by the pair kernel and the resolver)
+// ----------------------------------------------------------------------------
+// Builds Y = sum_c Xhat_ic conj(Xhat_jc) for this lane's bins t + 64m (row i
+// from LDS, row j from global), converts the half spectrum to the packed
+// half-length sequence Z[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/P}
+// (Y[k] - conj Y[M-k]) and runs the inverse wave FFT.  On return lane l,
+// register r holds c[2q] (x) and c[2q+1] (y), q = wave_out_index(l, r).
+template <int R>
+__device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
+                                             int xi_stride, const float2* __restrict__ xj,
+                                             long long xj_stride, int nch, const WaveTw<R>& tw,
+                                             cf prelane, float2* wl, int lane) {
+  constexpr int M = 64 * R;
+#pragma unroll
+  for (int m = 0; m < R; ++m) y[m] = cmk(0.f, 0.f);
+  float yN = 0.f;
+  for (int c = 0; c < nch; ++c) {
+    const float2* ai = xi + c * xi_stride;
+    const float2* bj = xj + c * xj_stride;
+    // groups of kGroup rows bound the loads in flight (and the registers)
+    constexpr int kGroup = R < 16 ? R : 16;
+#pragma unroll
+    for (int m0 = 0; m0 < R; m0 += kGroup) {
+#pragma unroll
+      for (int m = m0; m < m0 + kGroup; ++m) {
+        cf A = f2c(ai[lane + 64 * m]);
+        cf B = f2c(bj[lane + 64 * m]);
+        y[m].x = __builtin_fmaf(A.x, B.x, __builtin_fmaf(A.y, B.y, y[m].x));
+        y[m].y = __builtin_fmaf(A.y, B.x, __builtin_fmaf(-A.x, B.y, y[m].y));
+      }
+      asm volatile("" ::: "memory");
+    }
+    float2 aN = ai[M], bN = bj[M];
+    yN = __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
+  }
+  // c2r ignores the imaginary parts of the DC and Nyquist bins
+  if (lane == 0) y[0].y = 0.f;
+
+  const int src = (64 - lane) & 63;
+  cf saved = cmk(yN, 0.f);  // lane 0: original y[R - m]
+  sfor<0, R / 2>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    constexpr int mh = R - 1 - m;
+    cf a = y[m], b = y[mh];
+    cf am, bm;
+    am.x = __shfl(b.x, src, 64);
+    am.y = __shfl(b.y, src, 64);
+    bm.x = __shfl(a.x, src, 64);
+    bm.y = __shfl(a.y, src, 64);
+    if (lane == 0) {
+      am = saved;
+      bm = (m + 1 == mh) ? b : y[m + 1];
+    }
+    saved = b;
+    // twiddle e^{2 pi i (lane + 64 m)/P} = prelane * e^{2 pi i m / (2R)}
+    cf ta = cmul(prelane, cmk(TwC<m, 2 * R, 1>::c, TwC<m, 2 * R, 1>::s));
+    cf tb = cmul(prelane, cmk(TwC<mh, 2 * R, 1>::c, TwC<mh, 2 * R, 1>::s));
+    {
+      cf E = cmk(a.x + am.x, a.y - am.y);
+      cf D = cmk(a.x - am.x, a.y + am.y);
+      cf TD = cmul(ta, D);
+      y[m] = cmk(E.x - TD.y, E.y + TD.x);
+    }
+    {
+      cf E = cmk(b.x + bm.x, b.y - bm.y);
+      cf D = cmk(b.x - bm.x, b.y + bm.y);
+      cf TD = cmul(tb, D);
+      y[mh] = cmk(E.x - TD.y, E.y + TD.x);
+    }
+  });
+  wave_fft<R, 1>(y, tw, wl, lane);
+}
+
+// lag of (lane, slot) where slot = 2*reg + {0: x, 1: y}
+template <int R>
+__device__ __forceinline__ int slot_lag(int lane, int slot) {
+  constexpr int M = 64 * R;
+  const int reg = slot >> 1;
+  const int q = ((lane >> 3) + 8 * (reg >> 3)) + R * ((lane & 7) + 8 * (reg & 7));
+  const int n = 2 * q + (slot & 1);
+  return n <= M ? n : n - 2 * M;
+}
+
+// zero every value whose lag is outside [lo, hi]
+template <int R>
+__device__ __forceinline__ void apply_window(cf (&y)[R], int lane, int lo, int hi) {
+  sfor<0, R>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    const int l0 = slot_lag<R>(lane, 2 * r), l1 = slot_lag<R>(lane, 2 * r + 1);
+    if (l0 < lo || l0 > hi) y[r].x = 0.f;
+    if (l1 < lo || l1 > hi) y[r].y = 0.f;
+  });
+}
+
+struct Best {
+  float v;
+  int lag;
+};
+
+// max with ties to the smaller lag (and min likewise) across the wave
+__device__ __forceinline__ Best wave_best_max(Best b) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float ov = __shfl_xor(b.v, o, 64);
+    int ol = __shfl_xor(b.lag, o, 64);
+    if (ov > b.v || (ov == b.v && ol < b.lag)) { b.v = ov; b.lag = ol; }
+  }
+  return b;
+}
+__device__ __forceinline__ Best wave_best_min(Best b) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    float ov = __shfl_xor(b.v, o, 64);
+    int ol = __shfl_xor(b.lag, o, 64);
+    if (ov < b.v || (ov == b.v && ol < b.lag)) { b.v = ov; b.lag = ol; }
+  }
+  return b;
+}
+
+// Reference scan rule over the wave's values: strict > (<) starting from
+// (0, lag 0).  Returns wave-uniform results and whether another value lies
+// within tol of the winner (near tie, to be settled in FP64).
+template <int R>
+__device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol, float thr,
+                                          Best& pos, Best& neg, bool& tie_p, bool& tie_n) {
+  float bp = 0.f, bn = 0.f;
+  int sp = -1, sn = -1;
+  sfor<0, 2 * R>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
+    if (v > bp) { bp = v; sp = s; }
+    if (v < bn) { bn = v; sn = s; }
+  });
+  Best p{bp, sp < 0 ? 0x7fffffff : slot_lag<R>(lane, sp)};
+  Best q{bn, sn < 0 ? 0x7fffffff : slot_lag<R>(lane, sn)};
+  p = wave_best_max(p);
+  q = wave_best_min(q);
+  if (p.lag == 0x7fffffff) { p.v = 0.f; p.lag = 0; }
+  if (q.lag == 0x7fffffff) { q.v = 0.f; q.lag = 0; }
+  pos = p;
+  neg = q;
+  tie_p = false;
+  tie_n = false;
+  if (p.v > thr && p.v > 0.f) {
+    const float cut = p.v - tol;
+    int cnt = 0;
+    sfor<0, 2 * R>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
+      cnt += __popcll(__ballot(v >= cut));
+    });
+    tie_p = cnt > 1 || cut <= 0.f;
+  }
+  if (q.v < -thr && q.v < 0.f) {
+    const float cut = q.v + tol;
+    int cnt = 0;
+    sfor<0, 2 * R>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
+      cnt += __popcll(__ballot(v <= cut));
+    });
+    tie_n = cnt > 1 || cut >= 0.f;
+  }
+}
+
+__device__ __forceinline__ long long packed_index(int i, int j, int n) {
+  return (long long)i * n - (long long)i * (i - 1) / 2 + (j - i);
+}
+
+// ----------------------------------------------------------------------------
+// pair kernel: one workgroup = 4 waves per work item (row i, column chunk)
+// ----------------------------------------------------------------------------
+template <int R, bool FULL>
+__global__ __launch_bounds__(256, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
+  constexpr int M = 64 * R;
+  extern __shared__ float2 smem[];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int xs = M + 1;  // LDS row stride of the staged row spectra
+  float2* xi_lds = smem;
+  float2* wl = smem + a.n_channels * xs + wv * kWaveLds;
+
+  WaveTw<R> tw;
+  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+  const cf prelane = f2c(a.pre_lane[lane]);
+
+  for (int item = blockIdx.x; item < a.n_work; item += gridDim.x) {
+    const int2 wk = a.work[item];
+    const int i = wk.x;
+    const int jb = wk.y;
+    const int je = min((jb / a.chunk + 1) * a.chunk, a.n_events);  // end of jb's column chunk
+    __syncthreads();
+    const float2* src = a.spec + (long long)i * a.n_channels * a.spec_stride;
+    for (int c = 0; c < a.n_channels; ++c)
+      for (int k = threadIdx.x; k <= M; k += blockDim.x)
+        xi_lds[c * xs + k] = src[c * a.spec_stride + k];
+    __syncthreads();
+
+    for (int j = max(jb, i) + wv; j < je; j += 4) {
+      cf y[R];
+      const float2* xj = a.spec + (long long)j * a.n_channels * a.spec_stride;
+      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, tw, prelane, wl, lane);
+      if constexpr (!FULL) apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
+      Best pos, neg;
+      bool tp, tn;
+      scan_wave<R>(y, lane, a.tie_tol, a.threshold, pos, neg, tp, tn);
+      if (lane == 0) {
+        const long long idx = packed_index(i, j, a.n_events);
+        const bool kp = pos.v > a.threshold, kn = neg.v < -a.threshold;
+        a.vpos[idx] = kp ? pos.v : 0.f;
+        a.lpos[idx] = kp ? pos.lag : 0;
+        a.vneg[idx] = kn ? neg.v : 0.f;
+        a.lneg[idx] = kn ? neg.lag : 0;
+        if (tp || tn) {
+          unsigned slot = atomicAdd(a.tie_count, 1u);
+          if (slot < (unsigned)a.tie_cap) a.ties[slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0), 0);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// FP64 near-tie resolver: one wave per recorded pair
+// ----------------------------------------------------------------------------
+// Recomputes the pair's FP32 correlation, collects every in-window lag whose
+// value is within tol of the winner, evaluates those lags as FP64 direct
+// sums over the time-domain traces and keeps the largest (smallest) with
+// ties to the smaller lag; starting state (0, lag 0) as in the reference scan.
+__device__ double corr_f64(const float* __restrict__ ti, const float* __restrict__ tj, int len,
+                           int P, int lag, int lane) {
+  double s = 0.0;
+  for (int n = lane; n < len; n += 64) {
+    int u = n - lag;
+    u %= P;
+    if (u < 0) u += P;
+    if (u < len) s += (double)ti[n] * (double)tj[u];
+  }
+  return wave_sum_f64(s);
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
+  constexpr int M = 64 * R;
+  constexpr int P = 2 * M;
+  extern __shared__ float2 smem[];
+  const int lane = threadIdx.x;
+  float2* xi_l = smem;                            // n_channels * (M+1)
+  float2* wl = smem + a.n_channels * (M + 1);
+  WaveTw<R> tw;
+  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+  const cf prelane = f2c(a.pre_lane[lane]);
+  const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
+  for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
+    const int4 t = a.ties[rec];
+    const int i = t.x, j = t.y, side = t.z;
+    const float2* si = a.spec + (long long)i * a.n_channels * a.spec_stride;
+    for (int c = 0; c < a.n_channels; ++c)
+      for (int k = lane; k <= M; k += 64) xi_l[c * (M + 1) + k] = si[c * a.spec_stride + k];
+    wave_lds_fence();
+    cf y[R];
+    pair_inverse<R>(y, xi_l, M + 1, a.spec + (long long)j * a.n_channels * a.spec_stride,
+                    a.spec_stride, a.n_channels, tw, prelane, wl, lane);
+    apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
+    const long long idx = packed_index(i, j, a.n_events);
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!(side & (1 << sd))) continue;
+      const float sg = sd == 0 ? 1.f : -1.f;
+      // FP32 winner of this side
+      float bv = 0.f;
+      sfor<0, 2 * R>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const float v = sg * ((s & 1) ? y[s >> 1].y : y[s >> 1].x);
+        bv = fmaxf(bv, v);
+      });
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) bv = fmaxf(bv, __shfl_xor(bv, o, 64));
+      const float cut = bv - a.tie_tol;
+      double best = 0.0;
+      int best_lag = 0;
+      bool have = false;
+      sfor<0, 2 * R>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        const float v = sg * ((s & 1) ? y[s >> 1].y : y[s >> 1].x);
+        const int lg = slot_lag<R>(lane, s);
+        const bool inw = lg >= a.lag_lo && lg <= a.lag_hi;
+        unsigned long long mask = __ballot(inw && v >= cut && v > 0.f);
+        while (mask) {
+          const int src = __ffsll((long long)mask) - 1;
+          mask &= mask - 1;
+          const int lag = __shfl(lg, src, 64);
+          double acc = 0.0;
+          for (int c = 0; c < a.n_channels; ++c) {
+            const double ni = a.norms[(long long)i * a.n_channels + c];
+            const double nj = a.norms[(long long)j * a.n_channels + c];
+            if (!(ni > 0.0 && nj > 0.0)) continue;
+            const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+            const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+            acc += corr_f64(ti, tj, a.tr_len, P, lag, lane) / (ni * nj * (double)a.n_channels);
+          }
+          acc *= (double)sg;
+          if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
+            best = acc;
+            best_lag = lag;
+            have = true;
+          }
+        }
+      });
+      if (lane == 0) {
+        const float v = (float)best;
+        if (sd == 0) {
+          const bool keep = have && v > a.threshold;
+          a.vpos[idx] = keep ? v : 0.f;
+          a.lpos[idx] = keep ? best_lag : 0;
+        } else {
+          const bool keep = have && -v < -a.threshold;
+          a.vneg[idx] = keep ? -v : 0.f;
+          a.lneg[idx] = keep ? best_lag : 0;
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// launchers (this translation unit instantiates one R, given by SXC_R)
+// ----------------------------------------------------------------------------
+template <int R>
+hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
+  constexpr int M = 64 * R;
+  size_t lds = (size_t)(kWaveLds + M) * sizeof(float2);
+  hipError_t e = hipFuncSetAttribute((const void*)k_forward<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_forward<R>, dim3(a.n_rows), dim3(64), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int R, bool FULL>
+static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
+  size_t lds = pair_lds_bytes(R, a.n_channels);
+  hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_pairs<R, FULL>), dim3(grid), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_pairs_r(bool full, const PairArgs& a, int grid, hipStream_t st) {
+  return full ? launch_pairs_rf<R, true>(a, grid, st) : launch_pairs_rf<R, false>(a, grid, st);
+}
+
+template <int R>
+hipError_t launch_resolve_r(const ResolveArgs& a, int grid, hipStream_t st) {
+  size_t lds = (size_t)(a.n_channels * (64 * R + 1) + kWaveLds) * sizeof(float2);
+  hipError_t e = hipFuncSetAttribute((const void*)k_resolve<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_resolve<R>, dim3(grid), dim3(64), lds, st, a);
+  return hipGetLastError();
+}
+
+template hipError_t launch_forward_r<SXC_R>(const ForwardArgs&, hipStream_t);
+template hipError_t launch_pairs_r<SXC_R>(bool, const PairArgs&, int, hipStream_t);
+template hipError_t launch_resolve_r<SXC_R>(const ResolveArgs&, int, hipStream_t);
+
+#if SXC_R == 8
+hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+#endif
+
+}  // namespace sxc

@@ -1,0 +1,128 @@
+"""Device-resident all-pairs plans over the extended C ABI
+(include/seismic_xcorr.h).  torch is used only to own device memory and
+streams; all compute runs in libseismic_xcorr.so."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import SxcPlanDesc, SxcStats, SxcTiming, c_float_p, c_int_p, last_error, load_engine
+
+FORM_FFT_F32 = 0
+
+
+def _desc(n_events, n_channels, trace_len, padded_size, lag_lo, lag_hi, threshold, tie_tol,
+          demean, form):
+    return SxcPlanDesc(int(n_events), int(n_channels), int(trace_len), int(padded_size),
+                       int(lag_lo), int(lag_hi), float(threshold), float(tie_tol),
+                       1 if demean else 0, int(form))
+
+
+def n_packed(n_events: int) -> int:
+    return n_events * (n_events + 1) // 2
+
+
+class AllPairsPlan:
+    """All-pairs NCC of N events x C channels, FFT period P, lag window
+    [lag_lo, lag_hi] (circular mod P), packed upper-triangle outputs."""
+
+    def __init__(self, n_events, n_channels, trace_len, padded_size, lag_lo, lag_hi,
+                 threshold=0.0, tie_tol=0.0, demean=False, form=FORM_FFT_F32):
+        self.lib = load_engine()
+        self.n_events = int(n_events)
+        self.n_channels = int(n_channels)
+        self.padded_size = int(padded_size)
+        self.desc = _desc(n_events, n_channels, trace_len, padded_size, lag_lo, lag_hi,
+                          threshold, tie_tol, demean, form)
+        h = ctypes.c_void_p()
+        rc = self.lib.sxc_plan_create(ctypes.byref(self.desc), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_create failed ({rc}): {last_error(self.lib)}")
+        self.handle = h
+
+    @property
+    def n_pairs(self) -> int:
+        return n_packed(self.n_events)
+
+    def alloc_outputs(self, device="cuda"):
+        import torch
+        n = self.n_pairs
+        return (torch.empty(n, dtype=torch.float32, device=device),
+                torch.empty(n, dtype=torch.int32, device=device),
+                torch.empty(n, dtype=torch.float32, device=device),
+                torch.empty(n, dtype=torch.int32, device=device))
+
+    def execute(self, traces, in_len=None, out=None, stream=None):
+        """traces: CUDA float32 tensor [N, C, stride] (or [N, stride] for C=1)."""
+        import torch
+        if not (traces.is_cuda and traces.dtype == torch.float32 and traces.is_contiguous()):
+            raise ValueError("traces must be a contiguous float32 CUDA tensor")
+        stride = traces.shape[-1]
+        if traces.numel() != self.n_events * self.n_channels * stride:
+            raise ValueError("traces shape does not match the plan")
+        in_len = stride if in_len is None else int(in_len)
+        if out is None:
+            out = self.alloc_outputs(traces.device)
+        vp, lp, vn, ln = out
+        if stream is None:
+            stream = torch.cuda.current_stream(traces.device)
+        rc = self.lib.sxc_plan_execute(self.handle, traces.data_ptr(), stride, in_len,
+                                       vp.data_ptr(), lp.data_ptr(), vn.data_ptr(), ln.data_ptr(),
+                                       ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_execute failed ({rc}): {last_error(self.lib)}")
+        return out
+
+    def stats(self):
+        s = SxcStats()
+        rc = self.lib.sxc_plan_stats(self.handle, ctypes.byref(s))
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_stats failed ({rc}): {last_error(self.lib)}")
+        return {"pairs": s.pairs, "near_ties": s.near_ties, "tie_overflow": s.tie_overflow}
+
+    def enable_timing(self, on=True):
+        rc = self.lib.sxc_plan_enable_timing(self.handle, 1 if on else 0)
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_enable_timing failed ({rc}): {last_error(self.lib)}")
+
+    def timing(self):
+        t = SxcTiming()
+        rc = self.lib.sxc_plan_timing(self.handle, ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_timing failed ({rc}): {last_error(self.lib)}")
+        return {"executes": t.executes, "ms_forward": t.ms_forward, "ms_pairs": t.ms_pairs,
+                "ms_resolve": t.ms_resolve}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.sxc_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def allpairs_host(traces, padded_size, lag_lo, lag_hi, trace_len=None, threshold=0.0,
+                  tie_tol=0.0, demean=False, in_len=None):
+    """Host-memory convenience call.  traces: [N, C, stride] or [N, stride] float32."""
+    lib = load_engine()
+    tr = np.ascontiguousarray(traces, dtype=np.float32)
+    if tr.ndim == 2:
+        tr = tr[:, None, :]
+    n, c, stride = tr.shape
+    L = stride if trace_len is None else trace_len
+    d = _desc(n, c, L, padded_size, lag_lo, lag_hi, threshold, tie_tol, demean, FORM_FFT_F32)
+    npk = n_packed(n)
+    vp = np.zeros(npk, np.float32); lp = np.zeros(npk, np.int32)
+    vn = np.zeros(npk, np.float32); ln = np.zeros(npk, np.int32)
+    rc = lib.sxc_allpairs_host(ctypes.byref(d), tr.ctypes.data_as(c_float_p), stride,
+                               stride if in_len is None else int(in_len),
+                               vp.ctypes.data_as(c_float_p), lp.ctypes.data_as(c_int_p),
+                               vn.ctypes.data_as(c_float_p), ln.ctypes.data_as(c_int_p))
+    if rc != 0:
+        raise RuntimeError(f"sxc_allpairs_host failed ({rc}): {last_error(lib)}")
+    return vp, lp, vn, ln

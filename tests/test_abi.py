@@ -1,0 +1,91 @@
+"""CPU-side checks of the drop-in boundary: the libraries load, export every
+symbol include/*.h declares, the Python mirror prepares inputs exactly as the
+reference binding does, and argument errors are reported without touching a
+GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from seismic_amd import _lib, correlation_lib, multicuda_lib, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", txt)))
+
+
+@pytest.mark.parametrize("lib,header", [("libseismic_xcorr.so", "seismic_xcorr.h"),
+                                         ("correlation_cuda.so", "correlation_cuda.h"),
+                                         ("correlation_multicuda.so", "correlation_multicuda.h")])
+def test_exports(lib, header):
+    path = _lib.library_path(lib)
+    assert os.path.exists(path), "build the libraries first (python -m seismic_amd.build)"
+    h = ctypes.CDLL(path)
+    names = _declared(header)
+    assert names, header
+    for n in names:
+        assert hasattr(h, n), f"{lib} does not export {n}"
+    for n in _lib.EXPORTS[lib]:
+        assert n in names
+
+
+def test_prepare_matches_reference_rules():
+    ev, _ = synth.make_events(5, 1, 1500, seed=1)
+    traces = [e[0] for e in ev]
+    n, L, P, padded, rev, outs = correlation_lib.prepare(np.array(traces))
+    assert (n, L, P) == (5, 1500, 2048)          # P = 1 << L.bit_length()
+    assert np.array_equal(padded[:, :L], np.array(traces))
+    assert not padded[:, L:].any()
+    assert np.array_equal(rev[:, :L], np.array(traces)[:, ::-1])
+    assert all(o.shape == (15,) and not o.any() for o in outs)
+    # power-of-two lengths get a strictly larger period
+    assert correlation_lib.prepare(np.zeros((2, 1024), np.float32))[2] == 2048
+
+
+def test_prepare_accepts_trace_objects():
+    class Tr:
+        def __init__(self, d):
+            self.data = d
+    ev, _ = synth.make_events(3, 1, 700, seed=2)
+    objs = np.empty(3, dtype=object)
+    for i in range(3):
+        objs[i] = Tr(ev[i, 0])
+    n, L, P, padded, rev, outs = correlation_lib.prepare(objs)
+    assert (n, L, P) == (3, 700, 1024)
+    assert np.array_equal(padded[:, :700], ev[:, 0])
+
+
+@pytest.mark.parametrize("shift,L,P", [(600, 700, 1024), (-1, 1024, 2048)])
+def test_invalid_shift_reported(shift, L, P):
+    lib = correlation_lib.initialise_library()
+    n = 3
+    ev = np.zeros((n, P), np.float32)
+    outs = [np.full(6, 7, np.float32), np.full(6, 7, np.int32), np.full(6, 7, np.float32),
+            np.full(6, 7, np.int32)]
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    lib.correlationCUDA(fp(ev), fp(ev), n, L, shift, P, 1, 0.0, fp(outs[0]), ip(outs[1]),
+                        fp(outs[2]), ip(outs[3]))
+    assert b"shift" in lib.seismic_last_error()
+    assert all((o == 7).all() for o in outs)       # untouched on error
+
+
+def test_invalid_plan_reported():
+    from seismic_amd import engine
+    with pytest.raises(RuntimeError, match="padded_size"):
+        engine.AllPairsPlan(4, 1, 100, 1000, -10, 10)
+    with pytest.raises(RuntimeError, match="lag window"):
+        engine.AllPairsPlan(4, 1, 100, 1024, 10, -10)
+    with pytest.raises(RuntimeError, match="n_channels"):
+        engine.AllPairsPlan(4, 0, 100, 1024, -10, 10)
+
+
+def test_multicuda_binding_loads():
+    lib = multicuda_lib.initialise_library()
+    assert hasattr(lib, "correlationCUDA") and hasattr(lib, "initialiseCUDA")
