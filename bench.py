@@ -75,39 +75,42 @@ def main():
               "supported: the all-pairs job runs on a single GPU", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from seismic_amd import engine, synth
+    from seismic_amd.device import DeviceArray, Stream, synchronize
 
     N, C, L, P = args.n_events, 3, 2048, 4096
     lag_lo, lag_hi = -P // 2 + 1, P // 2
     log(f"bench: generating N={N} C={C} L={L}")
     ev, _ = synth.make_events(N, C, L, seed=20240601)
-    dev = torch.device("cuda:0")
-    tr = torch.from_numpy(ev).to(dev)
+    tr = DeviceArray.from_numpy(ev)
     plan = engine.AllPairsPlan(N, C, L, P, lag_lo, lag_hi, threshold=0.0)
-    outs = plan.alloc_outputs(dev)
-    stream = torch.cuda.current_stream(dev)
+    outs = plan.alloc_outputs()
+    stream = Stream()
 
     if args.copy_calibration:
-        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).uniform_()
-        b = torch.empty_like(a)
+        from seismic_amd._lib import load_engine
+        lib = load_engine()
+        a = DeviceArray(64 * 1024 * 1024, np.float32).zero_()
+        b = DeviceArray(64 * 1024 * 1024, np.float32)
+        import ctypes
         for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
+            lib.sxc_copy_kernel(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), a.nbytes, None)
+        synchronize()
         log("bench: copy calibration 3 x 256 MiB done")
 
     for _ in range(args.warmup):
         plan.execute(tr, out=outs, stream=stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     st = plan.stats()
     log(f"bench: warmup done, near ties per step {st['near_ties']}")
 
     plan.enable_timing(True)
-    torch.cuda.synchronize()
+    synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.execute(tr, out=outs, stream=stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
+    synchronize()
     t1 = time.perf_counter()
     tm = plan.timing()
     st = plan.stats()

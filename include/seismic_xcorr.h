@@ -94,6 +94,22 @@ int sxc_plan_enable_timing(sxc_plan* plan, int on);
 /* Synchronises the recorded events and returns the averages. */
 int sxc_plan_timing(sxc_plan* plan, sxc_timing* out);
 
+/* Device memory and streams of the library's own HIP runtime, so callers
+ * need no second GPU runtime in the process.  Copies are synchronous. */
+int sxc_device_count(int* count);
+int sxc_device_alloc(void** ptr, unsigned long long bytes);
+int sxc_device_free(void* ptr);
+int sxc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes);
+int sxc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes);
+int sxc_memset_d8(void* dst, int value, unsigned long long bytes);
+int sxc_stream_create(void** stream);
+int sxc_stream_destroy(void* stream);
+int sxc_stream_synchronize(void* stream);
+int sxc_device_synchronize(void);
+/* Streaming copy kernel (bytes a multiple of 16): the byte-counter reference
+ * used to scale profiler HBM counters. */
+int sxc_copy_kernel(void* dst, const void* src, unsigned long long bytes, void* stream);
+
 /* Host-memory convenience wrapper: copies traces in, runs, copies the packed
  * outputs back.  traces: [n_events][n_channels][trace_stride] float32. */
 int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,

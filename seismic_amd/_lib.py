@@ -48,7 +48,11 @@ class SxcTiming(ctypes.Structure):
 EXPORTS = {
     "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
                             "sxc_plan_execute", "sxc_plan_stats", "sxc_allpairs_host",
-                            "sxc_plan_enable_timing", "sxc_plan_timing"],
+                            "sxc_plan_enable_timing", "sxc_plan_timing", "sxc_device_count",
+                            "sxc_device_alloc", "sxc_device_free", "sxc_memcpy_h2d",
+                            "sxc_memcpy_d2h", "sxc_memset_d8", "sxc_stream_create",
+                            "sxc_stream_destroy", "sxc_stream_synchronize",
+                            "sxc_device_synchronize", "sxc_copy_kernel"],
     "correlation_cuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
     "correlation_multicuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
 }
@@ -85,6 +89,21 @@ def load_engine() -> ctypes.CDLL:
     lib.sxc_plan_enable_timing.argtypes = [vp, ctypes.c_int]
     lib.sxc_plan_timing.restype = ctypes.c_int
     lib.sxc_plan_timing.argtypes = [vp, ctypes.POINTER(SxcTiming)]
+    u64 = ctypes.c_ulonglong
+    for name, args in (("sxc_device_count", [c_int_p]),
+                       ("sxc_device_alloc", [ctypes.POINTER(vp), u64]),
+                       ("sxc_device_free", [vp]),
+                       ("sxc_memcpy_h2d", [vp, vp, u64]),
+                       ("sxc_memcpy_d2h", [vp, vp, u64]),
+                       ("sxc_memset_d8", [vp, ctypes.c_int, u64]),
+                       ("sxc_stream_create", [ctypes.POINTER(vp)]),
+                       ("sxc_stream_destroy", [vp]),
+                       ("sxc_stream_synchronize", [vp]),
+                       ("sxc_device_synchronize", []),
+                       ("sxc_copy_kernel", [vp, vp, u64, vp])):
+        f = getattr(lib, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
     lib.sxc_allpairs_host.restype = ctypes.c_int
     lib.sxc_allpairs_host.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, ctypes.c_longlong,
                                       ctypes.c_int, c_float_p, c_int_p, c_float_p, c_int_p]

@@ -498,6 +498,87 @@ int warm_up() {
 }  // namespace sxc
 
 extern "C" {
+
+int sxc_device_count(int* count) {
+  clear_error();
+  if (!count) { set_error("null argument"); return SXC_ERR_INVALID; }
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) { *count = 0; }
+  return SXC_OK;
+}
+
+int sxc_device_alloc(void** ptr, unsigned long long bytes) {
+  clear_error();
+  if (!ptr) { set_error("null argument"); return SXC_ERR_INVALID; }
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%llu) failed: %s", bytes, hipGetErrorString(e));
+    *ptr = nullptr;
+    return SXC_ERR_MEMORY;
+  }
+  return SXC_OK;
+}
+
+int sxc_device_free(void* ptr) {
+  clear_error();
+  if (ptr) HIP_CHECK(hipFree(ptr));
+  return SXC_OK;
+}
+
+int sxc_memcpy_h2d(void* dst, const void* src, unsigned long long bytes) {
+  clear_error();
+  if (bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SXC_OK;
+}
+
+int sxc_memcpy_d2h(void* dst, const void* src, unsigned long long bytes) {
+  clear_error();
+  if (bytes) HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SXC_OK;
+}
+
+int sxc_memset_d8(void* dst, int value, unsigned long long bytes) {
+  clear_error();
+  if (bytes) HIP_CHECK(hipMemset(dst, value, bytes));
+  return SXC_OK;
+}
+
+int sxc_stream_create(void** stream) {
+  clear_error();
+  if (!stream) { set_error("null argument"); return SXC_ERR_INVALID; }
+  hipStream_t s = nullptr;
+  HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return SXC_OK;
+}
+
+int sxc_stream_destroy(void* stream) {
+  clear_error();
+  if (stream) HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+  return SXC_OK;
+}
+
+int sxc_stream_synchronize(void* stream) {
+  clear_error();
+  HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return SXC_OK;
+}
+
+int sxc_copy_kernel(void* dst, const void* src, unsigned long long bytes, void* stream) {
+  clear_error();
+  if (!dst || !src || (bytes & 15)) { set_error("copy needs non-null pointers and bytes %% 16 == 0"); return SXC_ERR_INVALID; }
+  HIP_CHECK(launch_copy_f4((float4*)dst, (const float4*)src, (long long)(bytes / 16), (hipStream_t)stream));
+  return SXC_OK;
+}
+
+int sxc_device_synchronize(void) {
+  clear_error();
+  HIP_CHECK(hipDeviceSynchronize());
+  return SXC_OK;
+}
+
 void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
                         float threshold, float* vp, int* lp, float* vn, int* ln) {
   sxc::reference_call(events, n_events, event_length, shift, padded, threshold, vp, lp, vn, ln);

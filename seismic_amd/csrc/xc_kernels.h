@@ -80,6 +80,7 @@ hipError_t launch_forward(int R, const ForwardArgs& a, hipStream_t st);
 hipError_t launch_pairs(int R, bool full, const PairArgs& a, int grid, hipStream_t st);
 hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st);
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
+hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream_t st);
 
 inline size_t pair_lds_bytes(int R, int n_channels) {
   return (size_t)(n_channels * (64 * R + 1) + 4 * 576) * 8;

@@ -497,6 +497,21 @@ template hipError_t launch_pairs_r<SXC_R>(bool, const PairArgs&, int, hipStream_
 template hipError_t launch_resolve_r<SXC_R>(const ResolveArgs&, int, hipStream_t);
 
 #if SXC_R == 8
+// Plain streaming copy of n float4 (device bandwidth reference for profiling).
+__global__ __launch_bounds__(256) void k_copy_f4(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                 long long n) {
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);

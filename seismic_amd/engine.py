@@ -1,6 +1,6 @@
 """Device-resident all-pairs plans over the extended C ABI
-(include/seismic_xcorr.h).  torch is used only to own device memory and
-streams; all compute runs in libseismic_xcorr.so."""
+(include/seismic_xcorr.h).  Device memory comes from seismic_amd.device (the
+engine's own HIP runtime); all compute runs in libseismic_xcorr.so."""
 from __future__ import annotations
 
 import ctypes
@@ -45,31 +45,27 @@ class AllPairsPlan:
     def n_pairs(self) -> int:
         return n_packed(self.n_events)
 
-    def alloc_outputs(self, device="cuda"):
-        import torch
+    def alloc_outputs(self):
+        from .device import DeviceArray
         n = self.n_pairs
-        return (torch.empty(n, dtype=torch.float32, device=device),
-                torch.empty(n, dtype=torch.int32, device=device),
-                torch.empty(n, dtype=torch.float32, device=device),
-                torch.empty(n, dtype=torch.int32, device=device))
+        return (DeviceArray(n, np.float32), DeviceArray(n, np.int32),
+                DeviceArray(n, np.float32), DeviceArray(n, np.int32))
 
     def execute(self, traces, in_len=None, out=None, stream=None):
-        """traces: CUDA float32 tensor [N, C, stride] (or [N, stride] for C=1)."""
-        import torch
-        if not (traces.is_cuda and traces.dtype == torch.float32 and traces.is_contiguous()):
-            raise ValueError("traces must be a contiguous float32 CUDA tensor")
+        """traces: DeviceArray float32 [N, C, stride] (or [N, stride] for C=1).
+        Asynchronous on `stream` (a device.Stream; None = default stream)."""
+        if traces.dtype != np.float32:
+            raise ValueError("traces must be float32")
         stride = traces.shape[-1]
-        if traces.numel() != self.n_events * self.n_channels * stride:
+        if int(np.prod(traces.shape)) != self.n_events * self.n_channels * stride:
             raise ValueError("traces shape does not match the plan")
         in_len = stride if in_len is None else int(in_len)
         if out is None:
-            out = self.alloc_outputs(traces.device)
+            out = self.alloc_outputs()
         vp, lp, vn, ln = out
-        if stream is None:
-            stream = torch.cuda.current_stream(traces.device)
-        rc = self.lib.sxc_plan_execute(self.handle, traces.data_ptr(), stride, in_len,
-                                       vp.data_ptr(), lp.data_ptr(), vn.data_ptr(), ln.data_ptr(),
-                                       ctypes.c_void_p(stream.cuda_stream))
+        sh = stream.handle if stream is not None else 0
+        rc = self.lib.sxc_plan_execute(self.handle, traces.ptr, stride, in_len,
+                                       vp.ptr, lp.ptr, vn.ptr, ln.ptr, ctypes.c_void_p(sh))
         if rc != 0:
             raise RuntimeError(f"sxc_plan_execute failed ({rc}): {last_error(self.lib)}")
         return out

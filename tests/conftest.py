@@ -13,9 +13,10 @@ def pytest_configure(config):
 
 
 def gpu_available() -> bool:
+    """Device count from the engine's own runtime (never a second GPU runtime)."""
     try:
-        import torch
-        return torch.cuda.is_available()
+        from seismic_amd.device import device_count
+        return device_count() > 0
     except Exception:
         return False
 

@@ -133,15 +133,16 @@ def test_initialise_runtime():
 
 def test_extended_multichannel(oracle):
     """3 channels averaged per lag, P=4096 full window, device-resident plan."""
-    import torch
     from seismic_amd import engine
+    from seismic_amd.device import DeviceArray, synchronize
     n, C, L, P = 32, 3, 2048, 4096
     ev, _ = synth.make_events(n, C, L, seed=11)
     print("case multichannel N=32 C=3 L=2048", flush=True)
     plan = engine.AllPairsPlan(n, C, L, P, -P // 2 + 1, P // 2)
-    tr = torch.from_numpy(ev).cuda()
-    vp, lp, vn, ln = [t.cpu().numpy() for t in plan.execute(tr)]
-    torch.cuda.synchronize()
+    tr = DeviceArray.from_numpy(ev)
+    outs = plan.execute(tr)
+    synchronize()
+    vp, lp, vn, ln = [t.numpy() for t in outs]
     pi, pj = oracle.packed_pairs(n)
     ref = oracle.pairs_f64(ev, pi, pj, -P // 2 + 1, P // 2, P=P, circular=True)
     assert np.abs(vp - ref["pos"]).max() <= COEF_TOL
