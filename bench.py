@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n-events", type=int, default=4096)
-    ap.add_argument("--cpu-sample-events", type=int, default=1024)
+    ap.add_argument("--cpu-sample-events", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (scales PMC byte counters)")

@@ -121,7 +121,7 @@ struct sxc_plan {
   int2* work = nullptr;
   int4* ties = nullptr;
   unsigned* tie_count = nullptr;
-  float2 *fwd2 = nullptr, *fwd64 = nullptr, *inv2 = nullptr, *inv64 = nullptr, *post = nullptr, *pre = nullptr;
+  float2 *fwd2 = nullptr, *inv2 = nullptr, *post = nullptr, *pre = nullptr;
   hipStream_t last_stream = nullptr;
   int device = 0;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
@@ -134,38 +134,40 @@ static void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
-                  p->fwd2, p->fwd64, p->inv2, p->inv64, p->post, p->pre};
+                  p->fwd2, p->inv2, p->post, p->pre};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
 }
 
+// Twiddle tables in the device layout of TwLayout<R> (xc_fft.hpp):
+//   [a*64 + t] = W_M^(t*a), a < 8; [512 + b*64 + t] = W_M^(8*t*b), b < R/8;
+//   [512 + 64*R/8 + k*8 + t1] = W_64^(t1*k).
+static std::vector<float2> wave_tables(int R, int sign) {
+  const int M = 64 * R;
+  const int kB = 512, kW = 512 + 64 * (R / 8);
+  std::vector<float2> t(kW + 64);
+  for (int a = 0; a < 8; ++a)
+    for (int l = 0; l < 64; ++l) twiddle((long)l * a, M, sign, &t[a * 64 + l].x, &t[a * 64 + l].y);
+  for (int b = 0; b < R / 8; ++b)
+    for (int l = 0; l < 64; ++l) twiddle((long)8 * l * b, M, sign, &t[kB + b * 64 + l].x, &t[kB + b * 64 + l].y);
+  for (int k = 0; k < 8; ++k)
+    for (int l = 0; l < 8; ++l) twiddle((long)l * k, 64, sign, &t[kW + k * 8 + l].x, &t[kW + k * 8 + l].y);
+  return t;
+}
+
 static int upload_tables(sxc_plan* p) {
   const int R = p->R, M = p->M, P = p->P;
-  std::vector<float2> f2(R * 64), i2(R * 64), f64(64), i64(64), post(M + 1), pre(64);
-  for (int k = 0; k < R; ++k)
-    for (int t = 0; t < 64; ++t) {
-      twiddle((long)t * k, M, -1, &f2[k * 64 + t].x, &f2[k * 64 + t].y);
-      twiddle((long)t * k, M, +1, &i2[k * 64 + t].x, &i2[k * 64 + t].y);
-    }
-  for (int k = 0; k < 8; ++k)
-    for (int t = 0; t < 8; ++t) {
-      twiddle((long)t * k, 64, -1, &f64[k * 8 + t].x, &f64[k * 8 + t].y);
-      twiddle((long)t * k, 64, +1, &i64[k * 8 + t].x, &i64[k * 8 + t].y);
-    }
+  std::vector<float2> fw = wave_tables(R, -1), iw = wave_tables(R, +1), post(M + 1), pre(64);
   for (int k = 0; k <= M; ++k) twiddle(k, P, -1, &post[k].x, &post[k].y);
   for (int t = 0; t < 64; ++t) twiddle(t, P, +1, &pre[t].x, &pre[t].y);
   int rc;
-  if ((rc = dev_alloc(&p->fwd2, f2.size(), "twiddles"))) return rc;
-  if ((rc = dev_alloc(&p->inv2, i2.size(), "twiddles"))) return rc;
-  if ((rc = dev_alloc(&p->fwd64, 64, "twiddles"))) return rc;
-  if ((rc = dev_alloc(&p->inv64, 64, "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->fwd2, fw.size(), "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->inv2, iw.size(), "twiddles"))) return rc;
   if ((rc = dev_alloc(&p->post, post.size(), "twiddles"))) return rc;
   if ((rc = dev_alloc(&p->pre, 64, "twiddles"))) return rc;
-  HIP_CHECK(hipMemcpy(p->fwd2, f2.data(), f2.size() * sizeof(float2), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(p->inv2, i2.data(), i2.size() * sizeof(float2), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(p->fwd64, f64.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(p->inv64, i64.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->fwd2, fw.data(), fw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->inv2, iw.data(), iw.size() * sizeof(float2), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(p->post, post.data(), post.size() * sizeof(float2), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(p->pre, pre.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
   return SXC_OK;
@@ -259,8 +261,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   fa.out_stride = p->P;
   fa.spec = p->spec;
   fa.spec_stride = p->spec_stride;
-  fa.tab2 = p->fwd2;
-  fa.tab64 = p->fwd64;
+  fa.tw = p->fwd2;
   fa.post_tw = p->post;
   hipEvent_t* ev = nullptr;
   if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
@@ -281,8 +282,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   pa.lag_hi = d.lag_hi;
   pa.threshold = d.threshold;
   pa.tie_tol = p->tie_tol;
-  pa.tab2 = p->inv2;
-  pa.tab64 = p->inv64;
+  pa.tw = p->inv2;
   pa.pre_lane = p->pre;
   pa.vpos = d_vals_pos;
   pa.lpos = d_lags_pos;
@@ -314,8 +314,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   ra.lag_hi = d.lag_hi;
   ra.threshold = d.threshold;
   ra.tie_tol = p->tie_tol;
-  ra.tab2 = p->inv2;
-  ra.tab64 = p->inv64;
+  ra.tw = p->inv2;
   ra.pre_lane = p->pre;
   ra.vpos = d_vals_pos;
   ra.lpos = d_lags_pos;

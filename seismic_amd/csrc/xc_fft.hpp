@@ -17,6 +17,7 @@
 #include <stdint.h>
 #include <type_traits>
 
+
 namespace sxc {
 
 struct cf {
@@ -168,55 +169,54 @@ __device__ __forceinline__ void wave_lds_fence() {
 
 constexpr int kWaveLds = 576;  // float2 per wave for the chunked transposes
 
-// Per-lane twiddles a wave FFT needs, loaded once per kernel.
-//   step 2: W_M^(t*k2), k2 < R.  For R <= 16 all R values sit in registers;
-//           for larger R they are factored as A[k2 % 8] * B[k2 / 8] with
-//           A[a] = W_M^(t*a), B[b] = W_M^(8*t*b) to save registers.
-//   W_64^((t%8)*k), k < 8, between the two radix-8 passes.
+// Twiddle tables a wave FFT reads, staged once per workgroup in LDS.
+//   step 2: W_M^(t*k2) = A[k2 % 8] * B[k2 / 8] with A[a] = W_M^(t*a) and
+//           B[b] = W_M^(8*t*b); A is [8][64], B is [R/8][64] (B[0] = 1).
+//   w64:    W_64^(t1*k), [8 k][8 t1], between the two radix-8 passes.
+// Layout (float2): A at 0, B at 512, w64 at 512 + 64*R/8.
 template <int R>
-struct WaveTw {
-  static constexpr bool kFactored = R > 16;
-  static constexpr int kNA = kFactored ? 8 : R;
-  static constexpr int kNB = kFactored ? R / 8 : 1;
-  cf a[kNA];
-  cf b[kNB];
-  cf tw64[8];
-  __device__ __forceinline__ cf step2(int k) const {
-    if constexpr (kFactored) {
-      return (k >> 3) == 0 ? a[k & 7] : ((k & 7) == 0 ? b[k >> 3] : cmul(a[k & 7], b[k >> 3]));
-    } else {
-      return a[k];
-    }
-  }
+struct TwLayout {
+  static constexpr int kA = 0;
+  static constexpr int kB = 512;
+  static constexpr int kW64 = 512 + 64 * (R / 8);
+  static constexpr int kSize = kW64 + 64;
 };
 
-// Device tables written by the host (FP64 generated, rounded once):
-//   tab2[k2*64 + t] = W_M^(t*k2) for k2 < R   (rows 8b give the B factors)
-//   tab64[k*8 + t1] = W_64^(t1*k)
+// Global twiddle table written by the host in the same layout:
+//   tab[a*64 + t] = W_M^(t*a), a < 8; tab[512 + b*64 + t] = W_M^(8*t*b), b < R/8;
+//   tab[kW64 + k*8 + t1] = W_64^(t1*k).  FP64 generated, rounded once.
 template <int R>
-__device__ __forceinline__ void load_wave_tw(WaveTw<R>& w, const float2* __restrict__ tab2,
-                                             const float2* __restrict__ tab64, int lane) {
-#pragma unroll
-  for (int k = 0; k < WaveTw<R>::kNA; ++k) w.a[k] = f2c(tab2[k * 64 + lane]);
-  if constexpr (WaveTw<R>::kFactored) {
-#pragma unroll
-    for (int k = 0; k < WaveTw<R>::kNB; ++k) w.b[k] = f2c(tab2[8 * k * 64 + lane]);
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) w.tw64[k] = f2c(tab64[k * 8 + (lane & 7)]);
+__device__ __forceinline__ void stage_twiddles(float2* lds_tw, const float2* __restrict__ tab,
+                                               int tid, int nthreads) {
+  for (int k = tid; k < TwLayout<R>::kSize; k += nthreads) lds_tw[k] = tab[k];
 }
 
 // The four-step wave FFT (see file comment).  `lds` is this wave's private
-// kWaveLds-float2 buffer.
+// kWaveLds-float2 buffer, `tw` the staged twiddle tables.
 template <int R, int SIGN>
-__device__ __forceinline__ void wave_fft(cf (&d)[R], const WaveTw<R>& w, float2* lds, int lane) {
+__device__ __forceinline__ void wave_fft(cf (&d)[R], const float2* tw, float2* lds, int lane) {
   static_assert(R % 8 == 0, "R must be a multiple of 8");
+  using TL = TwLayout<R>;
   reg_fft<R, SIGN>(d);
+  {
+    cf A[8], B[R / 8];
 #pragma unroll
-  for (int k = 1; k < R; ++k) d[k] = cmul(d[k], w.step2(k));
+    for (int a = 1; a < 8; ++a) A[a] = f2c(tw[TL::kA + a * 64 + lane]);
+#pragma unroll
+    for (int b = 1; b < R / 8; ++b) B[b] = f2c(tw[TL::kB + b * 64 + lane]);
+#pragma unroll
+    for (int k = 1; k < R; ++k) {
+      const int a = k & 7, b = k >> 3;
+      const cf w = b == 0 ? A[a] : (a == 0 ? B[b] : cmul(A[a], B[b]));
+      d[k] = cmul(d[k], w);
+    }
+  }
 
   const int t1 = lane & 7;
   const int g = lane >> 3;
+  cf w64[8];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) w64[k] = f2c(tw[TL::kW64 + k * 8 + t1]);
   // transpose 1 + radix-8 over t2, chunk by chunk
   sfor<0, R / 8>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void wave_fft(cf (&d)[R], const WaveTw<R>& w, float2*
     for (int t2 = 0; t2 < 8; ++t2) e[t2] = f2c(lds[g * 72 + t1 + 8 * t2]);
     reg_fft<8, SIGN>(e);
 #pragma unroll
-    for (int k = 1; k < 8; ++k) e[k] = cmul(e[k], w.tw64[k]);
+    for (int k = 1; k < 8; ++k) e[k] = cmul(e[k], w64[k]);
 #pragma unroll
     for (int k = 0; k < 8; ++k) d[8 * s + k] = e[k];
   });

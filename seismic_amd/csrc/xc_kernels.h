@@ -18,8 +18,7 @@ struct ForwardArgs {
   long long out_stride;
   float2* spec;           // [n_rows][spec_stride] bins 0..M
   long long spec_stride;
-  const float2* tab2;     // forward four-step twiddles [R][64]
-  const float2* tab64;    // forward W_64 twiddles [8][8]
+  const float2* tw;       // forward twiddle tables (TwLayout)
   const float2* post_tw;  // exp(-2 pi i k / P), k = 0..M
 };
 
@@ -34,8 +33,7 @@ struct PairArgs {
   int lag_lo, lag_hi;     // scanned lags, inclusive
   float threshold;
   float tie_tol;
-  const float2* tab2;     // inverse four-step twiddles [R][64]
-  const float2* tab64;    // inverse W_64 twiddles [8][8]
+  const float2* tw;       // inverse twiddle tables (TwLayout)
   const float2* pre_lane; // exp(+2 pi i t / P), t < 64
   float* vpos;
   int* lpos;
@@ -58,8 +56,7 @@ struct ResolveArgs {
   int lag_lo, lag_hi;
   float threshold;
   float tie_tol;
-  const float2* tab2;
-  const float2* tab64;
+  const float2* tw;
   const float2* pre_lane;
   float* vpos;
   int* lpos;
@@ -82,8 +79,9 @@ hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st)
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
 hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream_t st);
 
+// twiddle tables (576 + 8R) + staged row spectra + 4 wave transpose buffers
 inline size_t pair_lds_bytes(int R, int n_channels) {
-  return (size_t)(n_channels * (64 * R + 1) + 4 * 576) * 8;
+  return (size_t)(576 + 8 * R + n_channels * (64 * R + 1) + 4 * 576) * 8;
 }
 
 }  // namespace sxc

@@ -59,13 +59,14 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   constexpr int M = 64 * R;
   constexpr int P = 2 * M;
   extern __shared__ float2 smem[];
-  float2* wl = smem;              // wave FFT transposes
-  float2* zl = smem + kWaveLds;   // natural-order Z, M entries
+  float2* twl = smem;                            // twiddle tables
+  float2* wl = smem + TwLayout<R>::kSize;        // wave FFT transposes
+  float2* zl = wl + kWaveLds;                    // natural-order Z, M entries
   const int lane = threadIdx.x;
   const int row = blockIdx.x;
   if (row >= a.n_rows) return;
-  WaveTw<R> tw;
-  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+  stage_twiddles<R>(twl, a.tw, lane, 64);
+  wave_lds_fence();
 
   const float* x = a.in + (long long)row * a.in_stride;
   // mean and energy of the first L samples, FP64, straight from memory
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   if (lane == 0) a.norms[row] = sqrt(E);
   const float scale = E > 0.0 ? (float)(1.0 / sqrt(E * (double)a.n_channels * (double)P)) : 0.f;
 
-  wave_fft<R, -1>(d, tw, wl, lane);
+  wave_fft<R, -1>(d, twl, wl, lane);
 #pragma unroll
   for (int r = 0; r < R; ++r) zl[wave_out_index<R>(lane, r)] = c2f(d[r]);
   wave_lds_fence();
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
 template <int R>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
-                                             long long xj_stride, int nch, const WaveTw<R>& tw,
+                                             long long xj_stride, int nch, const float2* tw,
                                              cf prelane, float2* wl, int lane) {
   constexpr int M = 64 * R;
 #pragma unroll
@@ -150,18 +151,23 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
   for (int c = 0; c < nch; ++c) {
     const float2* ai = xi + c * xi_stride;
     const float2* bj = xj + c * xj_stride;
-    // groups of kGroup rows bound the loads in flight (and the registers)
-    constexpr int kGroup = R < 16 ? R : 16;
+    // All of a group's global loads are issued before the compiler fence so
+    // they are in flight together (one L2 round trip per group, not per row).
+    constexpr int kGroup = R < 32 ? R : 32;
 #pragma unroll
     for (int m0 = 0; m0 < R; m0 += kGroup) {
+      float2 Bv[kGroup];
 #pragma unroll
-      for (int m = m0; m < m0 + kGroup; ++m) {
+      for (int u = 0; u < kGroup; ++u) Bv[u] = bj[lane + 64 * (m0 + u)];
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int m = m0 + u;
         cf A = f2c(ai[lane + 64 * m]);
-        cf B = f2c(bj[lane + 64 * m]);
+        cf B = f2c(Bv[u]);
         y[m].x = __builtin_fmaf(A.x, B.x, __builtin_fmaf(A.y, B.y, y[m].x));
         y[m].y = __builtin_fmaf(A.y, B.x, __builtin_fmaf(-A.x, B.y, y[m].y));
       }
-      asm volatile("" ::: "memory");
     }
     float2 aN = ai[M], bN = bj[M];
     yN = __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
@@ -180,9 +186,13 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     am.y = __shfl(b.y, src, 64);
     bm.x = __shfl(a.x, src, 64);
     bm.y = __shfl(a.y, src, 64);
-    if (lane == 0) {
-      am = saved;
-      bm = (m + 1 == mh) ? b : y[m + 1];
+    {
+      const cf nb = (m + 1 == mh) ? b : y[m + 1];
+      const bool l0 = lane == 0;
+      am.x = l0 ? saved.x : am.x;
+      am.y = l0 ? saved.y : am.y;
+      bm.x = l0 ? nb.x : bm.x;
+      bm.y = l0 ? nb.y : bm.y;
     }
     saved = b;
     // twiddle e^{2 pi i (lane + 64 m)/P} = prelane * e^{2 pi i m / (2R)}
@@ -310,11 +320,11 @@ __global__ __launch_bounds__(256, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int xs = M + 1;  // LDS row stride of the staged row spectra
-  float2* xi_lds = smem;
-  float2* wl = smem + a.n_channels * xs + wv * kWaveLds;
+  float2* twl = smem;
+  float2* xi_lds = smem + TwLayout<R>::kSize;
+  float2* wl = xi_lds + a.n_channels * xs + wv * kWaveLds;
 
-  WaveTw<R> tw;
-  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+  stage_twiddles<R>(twl, a.tw, threadIdx.x, blockDim.x);
   const cf prelane = f2c(a.pre_lane[lane]);
 
   for (int item = blockIdx.x; item < a.n_work; item += gridDim.x) {
@@ -332,7 +342,7 @@ __global__ __launch_bounds__(256, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
     for (int j = max(jb, i) + wv; j < je; j += 4) {
       cf y[R];
       const float2* xj = a.spec + (long long)j * a.n_channels * a.spec_stride;
-      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, tw, prelane, wl, lane);
+      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, twl, prelane, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
       Best pos, neg;
       bool tp, tn;
@@ -378,10 +388,10 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   constexpr int P = 2 * M;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
-  float2* xi_l = smem;                            // n_channels * (M+1)
-  float2* wl = smem + a.n_channels * (M + 1);
-  WaveTw<R> tw;
-  load_wave_tw<R>(tw, a.tab2, a.tab64, lane);
+  float2* twl = smem;
+  float2* xi_l = smem + TwLayout<R>::kSize;       // n_channels * (M+1)
+  float2* wl = xi_l + a.n_channels * (M + 1);
+  stage_twiddles<R>(twl, a.tw, lane, 64);
   const cf prelane = f2c(a.pre_lane[lane]);
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
   for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
     wave_lds_fence();
     cf y[R];
     pair_inverse<R>(y, xi_l, M + 1, a.spec + (long long)j * a.n_channels * a.spec_stride,
-                    a.spec_stride, a.n_channels, tw, prelane, wl, lane);
+                    a.spec_stride, a.n_channels, twl, prelane, wl, lane);
     apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
     const long long idx = packed_index(i, j, a.n_events);
     for (int sd = 0; sd < 2; ++sd) {
@@ -461,7 +471,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
 template <int R>
 hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
   constexpr int M = 64 * R;
-  size_t lds = (size_t)(kWaveLds + M) * sizeof(float2);
+  size_t lds = (size_t)(TwLayout<R>::kSize + kWaveLds + M) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_forward<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_forward<R>, dim3(a.n_rows), dim3(64), lds, st, a);
@@ -471,6 +481,7 @@ hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
 template <int R, bool FULL>
 static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
   size_t lds = pair_lds_bytes(R, a.n_channels);
+  static_assert(TwLayout<R>::kSize == 576 + 8 * R, "twiddle layout");
   hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -485,7 +496,7 @@ hipError_t launch_pairs_r(bool full, const PairArgs& a, int grid, hipStream_t st
 
 template <int R>
 hipError_t launch_resolve_r(const ResolveArgs& a, int grid, hipStream_t st) {
-  size_t lds = (size_t)(a.n_channels * (64 * R + 1) + kWaveLds) * sizeof(float2);
+  size_t lds = (size_t)(TwLayout<R>::kSize + a.n_channels * (64 * R + 1) + kWaveLds) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_resolve<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resolve<R>, dim3(grid), dim3(64), lds, st, a);
