@@ -112,7 +112,7 @@ struct sxc_plan {
   bool full = false;
   long long spec_stride = 0;  // float2 per (event, channel) row
   long long n_pairs = 0;
-  int n_work = 0, chunk = 32, tie_cap = 0;
+  int n_work = 0, chunk = 128, tie_cap = 0;  // chunk: columns per work item
   float tie_tol = 1e-5f;
   // device workspace
   float2* spec = nullptr;
@@ -121,7 +121,8 @@ struct sxc_plan {
   int2* work = nullptr;
   int4* ties = nullptr;
   unsigned* tie_count = nullptr;
-  float2 *fwd2 = nullptr, *inv2 = nullptr, *post = nullptr, *pre = nullptr;
+  float2 *fwd2 = nullptr, *inv2 = nullptr, *post = nullptr;
+  unsigned long long* winmask = nullptr;
   hipStream_t last_stream = nullptr;
   int device = 0;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
@@ -134,42 +135,59 @@ static void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
-                  p->fwd2, p->inv2, p->post, p->pre};
+                  p->fwd2, p->inv2, p->post, p->winmask};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
 }
 
 // Twiddle tables in the device layout of TwLayout<R> (xc_fft.hpp):
-//   [a*64 + t] = W_M^(t*a), a < 8; [512 + b*64 + t] = W_M^(8*t*b), b < R/8;
-//   [512 + 64*R/8 + k*8 + t1] = W_64^(t1*k).
+//   [k2*64 + t] = W_M^(t*k2), k2 < R;  [64R + k*8 + t1] = W_64^(t1*k);
+//   inverse only: [64R + 64 + m*64 + t] = exp(+2 pi i (t + 64 m) / P).
 static std::vector<float2> wave_tables(int R, int sign) {
-  const int M = 64 * R;
-  const int kB = 512, kW = 512 + 64 * (R / 8);
-  std::vector<float2> t(kW + 64);
-  for (int a = 0; a < 8; ++a)
-    for (int l = 0; l < 64; ++l) twiddle((long)l * a, M, sign, &t[a * 64 + l].x, &t[a * 64 + l].y);
-  for (int b = 0; b < R / 8; ++b)
-    for (int l = 0; l < 64; ++l) twiddle((long)8 * l * b, M, sign, &t[kB + b * 64 + l].x, &t[kB + b * 64 + l].y);
+  const int M = 64 * R, P = 2 * M;
+  const int kW = 64 * R, kPre = 64 * R + 64;
+  std::vector<float2> t(sign > 0 ? kPre + 64 * R : kPre);
+  for (int k = 0; k < R; ++k)
+    for (int l = 0; l < 64; ++l) twiddle((long)l * k, M, sign, &t[k * 64 + l].x, &t[k * 64 + l].y);
   for (int k = 0; k < 8; ++k)
     for (int l = 0; l < 8; ++l) twiddle((long)l * k, 64, sign, &t[kW + k * 8 + l].x, &t[kW + k * 8 + l].y);
+  if (sign > 0)
+    for (int m = 0; m < R; ++m)
+      for (int l = 0; l < 64; ++l) twiddle(l + 64L * m, P, +1, &t[kPre + m * 64 + l].x, &t[kPre + m * 64 + l].y);
   return t;
+}
+
+// Lag held by (lane, slot) after the inverse wave FFT: slot = 2*reg + {re, im},
+// reg = 8s + k holds frequency index q = (lane/8 + 8s) + R*(lane%8 + 8k),
+// i.e. samples 2q and 2q+1 of the period-P correlation.
+static int slot_lag_host(int R, int lane, int slot) {
+  const int M = 64 * R;
+  const int reg = slot >> 1;
+  const int q = ((lane >> 3) + 8 * (reg >> 3)) + R * ((lane & 7) + 8 * (reg & 7));
+  const int n = 2 * q + (slot & 1);
+  return n <= M ? n : n - 2 * M;
 }
 
 static int upload_tables(sxc_plan* p) {
   const int R = p->R, M = p->M, P = p->P;
-  std::vector<float2> fw = wave_tables(R, -1), iw = wave_tables(R, +1), post(M + 1), pre(64);
+  std::vector<float2> fw = wave_tables(R, -1), iw = wave_tables(R, +1), post(M + 1);
   for (int k = 0; k <= M; ++k) twiddle(k, P, -1, &post[k].x, &post[k].y);
-  for (int t = 0; t < 64; ++t) twiddle(t, P, +1, &pre[t].x, &pre[t].y);
+  std::vector<unsigned long long> wm(2 * R, 0ull);
+  for (int s = 0; s < 2 * R; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const int lag = slot_lag_host(R, l, s);
+      if (lag >= p->d.lag_lo && lag <= p->d.lag_hi) wm[s] |= 1ull << l;
+    }
   int rc;
   if ((rc = dev_alloc(&p->fwd2, fw.size(), "twiddles"))) return rc;
   if ((rc = dev_alloc(&p->inv2, iw.size(), "twiddles"))) return rc;
   if ((rc = dev_alloc(&p->post, post.size(), "twiddles"))) return rc;
-  if ((rc = dev_alloc(&p->pre, 64, "twiddles"))) return rc;
+  if ((rc = dev_alloc(&p->winmask, wm.size(), "window masks"))) return rc;
   HIP_CHECK(hipMemcpy(p->fwd2, fw.data(), fw.size() * sizeof(float2), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(p->inv2, iw.data(), iw.size() * sizeof(float2), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(p->post, post.data(), post.size() * sizeof(float2), hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(p->pre, pre.data(), 64 * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(p->winmask, wm.data(), wm.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
   return SXC_OK;
 }
 
@@ -283,7 +301,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   pa.threshold = d.threshold;
   pa.tie_tol = p->tie_tol;
   pa.tw = p->inv2;
-  pa.pre_lane = p->pre;
+  pa.winmask = p->winmask;
   pa.vpos = d_vals_pos;
   pa.lpos = d_lags_pos;
   pa.vneg = d_vals_neg;
@@ -291,7 +309,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   pa.ties = p->ties;
   pa.tie_count = p->tie_count;
   pa.tie_cap = p->tie_cap;
-  const int grid = std::min(p->n_work, 8192);
+  const int grid = std::min(p->n_work, 4096);
   HIP_CHECK(launch_pairs(p->R, p->full, pa, grid, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
 
@@ -315,7 +333,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   ra.threshold = d.threshold;
   ra.tie_tol = p->tie_tol;
   ra.tw = p->inv2;
-  ra.pre_lane = p->pre;
+  ra.winmask = p->winmask;
   ra.vpos = d_vals_pos;
   ra.lpos = d_lags_pos;
   ra.vneg = d_vals_neg;
@@ -323,7 +341,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   ra.ties = p->ties;
   ra.tie_count = p->tie_count;
   ra.tie_cap = p->tie_cap;
-  HIP_CHECK(launch_resolve(p->R, ra, 512, st));
+  HIP_CHECK(launch_resolve(p->R, ra, 2048, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
 }

@@ -169,26 +169,25 @@ __device__ __forceinline__ void wave_lds_fence() {
 
 constexpr int kWaveLds = 576;  // float2 per wave for the chunked transposes
 
-// Twiddle tables a wave FFT reads, staged once per workgroup in LDS.
-//   step 2: W_M^(t*k2) = A[k2 % 8] * B[k2 / 8] with A[a] = W_M^(t*a) and
-//           B[b] = W_M^(8*t*b); A is [8][64], B is [R/8][64] (B[0] = 1).
-//   w64:    W_64^(t1*k), [8 k][8 t1], between the two radix-8 passes.
-// Layout (float2): A at 0, B at 512, w64 at 512 + 64*R/8.
+// Twiddle tables a wave FFT reads, staged once per workgroup in LDS
+// (float2, FP64 generated on the host and rounded once):
+//   step 2:  tw[k2*64 + t] = W_M^(t*k2), k2 < R           (R*64 entries)
+//   w64:     tw[kW64 + k*8 + t1] = W_64^(t1*k), k, t1 < 8  (64 entries)
+//   pre:     tw[kPre + m*64 + t] = exp(+2 pi i (t + 64 m) / P), m < R
+//            (half-length C2R packing; inverse tables only)
 template <int R>
 struct TwLayout {
-  static constexpr int kA = 0;
-  static constexpr int kB = 512;
-  static constexpr int kW64 = 512 + 64 * (R / 8);
-  static constexpr int kSize = kW64 + 64;
+  static constexpr int kS2 = 0;
+  static constexpr int kW64 = 64 * R;
+  static constexpr int kPre = 64 * R + 64;
+  static constexpr int kFft = kPre;          // entries a wave FFT needs
+  static constexpr int kInv = kPre + 64 * R; // entries the pair kernels need
 };
 
-// Global twiddle table written by the host in the same layout:
-//   tab[a*64 + t] = W_M^(t*a), a < 8; tab[512 + b*64 + t] = W_M^(8*t*b), b < R/8;
-//   tab[kW64 + k*8 + t1] = W_64^(t1*k).  FP64 generated, rounded once.
 template <int R>
 __device__ __forceinline__ void stage_twiddles(float2* lds_tw, const float2* __restrict__ tab,
-                                               int tid, int nthreads) {
-  for (int k = tid; k < TwLayout<R>::kSize; k += nthreads) lds_tw[k] = tab[k];
+                                               int n, int tid, int nthreads) {
+  for (int k = tid; k < n; k += nthreads) lds_tw[k] = tab[k];
 }
 
 // The four-step wave FFT (see file comment).  `lds` is this wave's private
@@ -198,19 +197,8 @@ __device__ __forceinline__ void wave_fft(cf (&d)[R], const float2* tw, float2* l
   static_assert(R % 8 == 0, "R must be a multiple of 8");
   using TL = TwLayout<R>;
   reg_fft<R, SIGN>(d);
-  {
-    cf A[8], B[R / 8];
 #pragma unroll
-    for (int a = 1; a < 8; ++a) A[a] = f2c(tw[TL::kA + a * 64 + lane]);
-#pragma unroll
-    for (int b = 1; b < R / 8; ++b) B[b] = f2c(tw[TL::kB + b * 64 + lane]);
-#pragma unroll
-    for (int k = 1; k < R; ++k) {
-      const int a = k & 7, b = k >> 3;
-      const cf w = b == 0 ? A[a] : (a == 0 ? B[b] : cmul(A[a], B[b]));
-      d[k] = cmul(d[k], w);
-    }
-  }
+  for (int k = 1; k < R; ++k) d[k] = cmul(d[k], f2c(tw[TL::kS2 + k * 64 + lane]));
 
   const int t1 = lane & 7;
   const int g = lane >> 3;

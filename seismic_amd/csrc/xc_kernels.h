@@ -33,8 +33,8 @@ struct PairArgs {
   int lag_lo, lag_hi;     // scanned lags, inclusive
   float threshold;
   float tie_tol;
-  const float2* tw;       // inverse twiddle tables (TwLayout)
-  const float2* pre_lane; // exp(+2 pi i t / P), t < 64
+  const float2* tw;       // inverse twiddle tables (TwLayout, incl. the C2R packing table)
+  const unsigned long long* winmask;  // [2R] lane masks of in-window slots
   float* vpos;
   int* lpos;
   float* vneg;
@@ -57,7 +57,7 @@ struct ResolveArgs {
   float threshold;
   float tie_tol;
   const float2* tw;
-  const float2* pre_lane;
+  const unsigned long long* winmask;
   float* vpos;
   int* lpos;
   float* vneg;
@@ -79,9 +79,9 @@ hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st)
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
 hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream_t st);
 
-// twiddle tables (576 + 8R) + staged row spectra + 4 wave transpose buffers
-inline size_t pair_lds_bytes(int R, int n_channels) {
-  return (size_t)(576 + 8 * R + n_channels * (64 * R + 1) + 4 * 576) * 8;
+// twiddle tables (128R + 64) + staged row spectra + one transpose buffer per wave
+inline size_t pair_lds_bytes(int R, int n_channels, int waves) {
+  return (size_t)(128 * R + 64 + n_channels * (64 * R + 1) + waves * 576) * 8;
 }
 
 }  // namespace sxc

@@ -60,12 +60,12 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   constexpr int P = 2 * M;
   extern __shared__ float2 smem[];
   float2* twl = smem;                            // twiddle tables
-  float2* wl = smem + TwLayout<R>::kSize;        // wave FFT transposes
+  float2* wl = smem + TwLayout<R>::kFft;         // wave FFT transposes
   float2* zl = wl + kWaveLds;                    // natural-order Z, M entries
   const int lane = threadIdx.x;
   const int row = blockIdx.x;
   if (row >= a.n_rows) return;
-  stage_twiddles<R>(twl, a.tw, lane, 64);
+  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kFft, lane, 64);
   wave_lds_fence();
 
   const float* x = a.in + (long long)row * a.in_stride;
@@ -139,11 +139,11 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
 // half-length sequence Z[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/P}
 // (Y[k] - conj Y[M-k]) and runs the inverse wave FFT.  On return lane l,
 // register r holds c[2q] (x) and c[2q+1] (y), q = wave_out_index(l, r).
-template <int R>
+template <int R, bool XI_GLOBAL = false>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
                                              long long xj_stride, int nch, const float2* tw,
-                                             cf prelane, float2* wl, int lane) {
+                                             float2* wl, int lane) {
   constexpr int M = 64 * R;
 #pragma unroll
   for (int m = 0; m < R; ++m) y[m] = cmk(0.f, 0.f);
@@ -157,13 +157,18 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
 #pragma unroll
     for (int m0 = 0; m0 < R; m0 += kGroup) {
       float2 Bv[kGroup];
+      float2 Av[XI_GLOBAL ? kGroup : 1];
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) Bv[u] = bj[lane + 64 * (m0 + u)];
+      if constexpr (XI_GLOBAL) {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) Av[u] = ai[lane + 64 * (m0 + u)];
+      }
       asm volatile("" ::: "memory");
 #pragma unroll
       for (int u = 0; u < kGroup; ++u) {
         const int m = m0 + u;
-        cf A = f2c(ai[lane + 64 * m]);
+        cf A = f2c(XI_GLOBAL ? Av[XI_GLOBAL ? u : 0] : ai[lane + 64 * m]);
         cf B = f2c(Bv[u]);
         y[m].x = __builtin_fmaf(A.x, B.x, __builtin_fmaf(A.y, B.y, y[m].x));
         y[m].y = __builtin_fmaf(A.y, B.x, __builtin_fmaf(-A.x, B.y, y[m].y));
@@ -176,7 +181,15 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
   if (lane == 0) y[0].y = 0.f;
 
   const int src = (64 - lane) & 63;
+  const float2* pre = tw + TwLayout<R>::kPre + lane;
   cf saved = cmk(yN, 0.f);  // lane 0: original y[R - m]
+  // Z = E + i T D with E = Y + conj(Ym), D = Y - conj(Ym), T = e^{2 pi i k/P}
+  auto zform = [](cf a, cf am, cf t) {
+    const float ex = a.x + am.x, ey = a.y - am.y;
+    const float dx = a.x - am.x, dy = a.y + am.y;
+    return cmk(__builtin_fmaf(-t.x, dy, __builtin_fmaf(-t.y, dx, ex)),
+               __builtin_fmaf(t.x, dx, __builtin_fmaf(-t.y, dy, ey)));
+  };
   sfor<0, R / 2>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     constexpr int mh = R - 1 - m;
@@ -195,28 +208,15 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
       bm.y = l0 ? nb.y : bm.y;
     }
     saved = b;
-    // twiddle e^{2 pi i (lane + 64 m)/P} = prelane * e^{2 pi i m / (2R)}
-    cf ta = cmul(prelane, cmk(TwC<m, 2 * R, 1>::c, TwC<m, 2 * R, 1>::s));
-    cf tb = cmul(prelane, cmk(TwC<mh, 2 * R, 1>::c, TwC<mh, 2 * R, 1>::s));
-    {
-      cf E = cmk(a.x + am.x, a.y - am.y);
-      cf D = cmk(a.x - am.x, a.y + am.y);
-      cf TD = cmul(ta, D);
-      y[m] = cmk(E.x - TD.y, E.y + TD.x);
-    }
-    {
-      cf E = cmk(b.x + bm.x, b.y - bm.y);
-      cf D = cmk(b.x - bm.x, b.y + bm.y);
-      cf TD = cmul(tb, D);
-      y[mh] = cmk(E.x - TD.y, E.y + TD.x);
-    }
+    y[m] = zform(a, am, f2c(pre[m * 64]));
+    y[mh] = zform(b, bm, f2c(pre[mh * 64]));
   });
   wave_fft<R, 1>(y, tw, wl, lane);
 }
 
 // lag of (lane, slot) where slot = 2*reg + {0: x, 1: y}
 template <int R>
-__device__ __forceinline__ int slot_lag(int lane, int slot) {
+__host__ __device__ __forceinline__ int slot_lag(int lane, int slot) {
   constexpr int M = 64 * R;
   const int reg = slot >> 1;
   const int q = ((lane >> 3) + 8 * (reg >> 3)) + R * ((lane & 7) + 8 * (reg & 7));
@@ -224,14 +224,17 @@ __device__ __forceinline__ int slot_lag(int lane, int slot) {
   return n <= M ? n : n - 2 * M;
 }
 
-// zero every value whose lag is outside [lo, hi]
+// Zero every value whose lag is outside the window.  winmask[s] has bit l
+// set when slot s of lane l is inside; it is wave-uniform, so it is read with
+// scalar loads and applied with one select per value.
 template <int R>
-__device__ __forceinline__ void apply_window(cf (&y)[R], int lane, int lo, int hi) {
+__device__ __forceinline__ void apply_window(cf (&y)[R], const unsigned long long* __restrict__ winmask,
+                                             int lane) {
   sfor<0, R>([&](auto rc) {
     constexpr int r = decltype(rc)::value;
-    const int l0 = slot_lag<R>(lane, 2 * r), l1 = slot_lag<R>(lane, 2 * r + 1);
-    if (l0 < lo || l0 > hi) y[r].x = 0.f;
-    if (l1 < lo || l1 > hi) y[r].y = 0.f;
+    const unsigned long long m0 = winmask[2 * r], m1 = winmask[2 * r + 1];
+    y[r].x = ((m0 >> lane) & 1ull) ? y[r].x : 0.f;
+    y[r].y = ((m1 >> lane) & 1ull) ? y[r].y : 0.f;
   });
 }
 
@@ -240,70 +243,51 @@ struct Best {
   int lag;
 };
 
-// max with ties to the smaller lag (and min likewise) across the wave
-__device__ __forceinline__ Best wave_best_max(Best b) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    float ov = __shfl_xor(b.v, o, 64);
-    int ol = __shfl_xor(b.lag, o, 64);
-    if (ov > b.v || (ov == b.v && ol < b.lag)) { b.v = ov; b.lag = ol; }
-  }
-  return b;
-}
-__device__ __forceinline__ Best wave_best_min(Best b) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    float ov = __shfl_xor(b.v, o, 64);
-    int ol = __shfl_xor(b.lag, o, 64);
-    if (ov < b.v || (ov == b.v && ol < b.lag)) { b.v = ov; b.lag = ol; }
-  }
-  return b;
-}
-
 // Reference scan rule over the wave's values: strict > (<) starting from
-// (0, lag 0).  Returns wave-uniform results and whether another value lies
-// within tol of the winner (near tie, to be settled in FP64).
+// (0, lag 0), first occurrence.  Pass 1 finds the exact extrema (max3/min3);
+// pass 2 compares every value with extremum -/+ tol: the ballot counts are
+// summed on the scalar unit, and the last slot whose ballot is non-zero
+// (with its lane mask) locates the extremum.  More than one value within tol
+// flags a near tie, which the FP64 resolver settles.
 template <int R>
 __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol, float thr,
                                           Best& pos, Best& neg, bool& tie_p, bool& tie_n) {
-  float bp = 0.f, bn = 0.f;
-  int sp = -1, sn = -1;
-  sfor<0, 2 * R>([&](auto sc) {
-    constexpr int s = decltype(sc)::value;
-    const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
-    if (v > bp) { bp = v; sp = s; }
-    if (v < bn) { bn = v; sn = s; }
+  float mx = 0.f, mn = 0.f;
+  sfor<0, R>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    mx = fmaxf(mx, fmaxf(y[r].x, y[r].y));
+    mn = fminf(mn, fminf(y[r].x, y[r].y));
   });
-  Best p{bp, sp < 0 ? 0x7fffffff : slot_lag<R>(lane, sp)};
-  Best q{bn, sn < 0 ? 0x7fffffff : slot_lag<R>(lane, sn)};
-  p = wave_best_max(p);
-  q = wave_best_min(q);
-  if (p.lag == 0x7fffffff) { p.v = 0.f; p.lag = 0; }
-  if (q.lag == 0x7fffffff) { q.v = 0.f; q.lag = 0; }
-  pos = p;
-  neg = q;
-  tie_p = false;
-  tie_n = false;
-  if (p.v > thr && p.v > 0.f) {
-    const float cut = p.v - tol;
-    int cnt = 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    mn = fminf(mn, __shfl_xor(mn, o, 64));
+  }
+  const bool want_p = mx > thr && mx > 0.f;
+  const bool want_n = mn < -thr && mn < 0.f;
+  const float cp = mx - tol, cn = mn + tol;
+  int cnt_p = 0, cnt_n = 0, slot_p = -1, slot_n = -1;
+  unsigned long long msk_p = 0, msk_n = 0;
+  if (want_p || want_n) {
     sfor<0, 2 * R>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
       const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
-      cnt += __popcll(__ballot(v >= cut));
+      const unsigned long long bp = __ballot(v >= cp);
+      const unsigned long long bn = __ballot(v <= cn);
+      cnt_p += __popcll(bp);
+      cnt_n += __popcll(bn);
+      slot_p = bp ? s : slot_p;
+      msk_p = bp ? bp : msk_p;
+      slot_n = bn ? s : slot_n;
+      msk_n = bn ? bn : msk_n;
     });
-    tie_p = cnt > 1 || cut <= 0.f;
   }
-  if (q.v < -thr && q.v < 0.f) {
-    const float cut = q.v + tol;
-    int cnt = 0;
-    sfor<0, 2 * R>([&](auto sc) {
-      constexpr int s = decltype(sc)::value;
-      const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
-      cnt += __popcll(__ballot(v <= cut));
-    });
-    tie_n = cnt > 1 || cut >= 0.f;
-  }
+  pos.v = want_p ? mx : (mx > 0.f ? mx : 0.f);
+  pos.lag = (want_p && slot_p >= 0) ? slot_lag<R>(__ffsll((long long)msk_p) - 1, slot_p) : 0;
+  neg.v = want_n ? mn : (mn < 0.f ? mn : 0.f);
+  neg.lag = (want_n && slot_n >= 0) ? slot_lag<R>(__ffsll((long long)msk_n) - 1, slot_n) : 0;
+  tie_p = want_p && (cnt_p > 1 || cp <= 0.f);
+  tie_n = want_n && (cnt_n > 1 || cn >= 0.f);
 }
 
 __device__ __forceinline__ long long packed_index(int i, int j, int n) {
@@ -313,19 +297,18 @@ __device__ __forceinline__ long long packed_index(int i, int j, int n) {
 // ----------------------------------------------------------------------------
 // pair kernel: one workgroup = 4 waves per work item (row i, column chunk)
 // ----------------------------------------------------------------------------
-template <int R, bool FULL>
-__global__ __launch_bounds__(256, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
+template <int R, bool FULL, int kWaves>
+__global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
   constexpr int M = 64 * R;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int xs = M + 1;  // LDS row stride of the staged row spectra
   float2* twl = smem;
-  float2* xi_lds = smem + TwLayout<R>::kSize;
+  float2* xi_lds = smem + TwLayout<R>::kInv;
   float2* wl = xi_lds + a.n_channels * xs + wv * kWaveLds;
 
-  stage_twiddles<R>(twl, a.tw, threadIdx.x, blockDim.x);
-  const cf prelane = f2c(a.pre_lane[lane]);
+  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kInv, threadIdx.x, blockDim.x);
 
   for (int item = blockIdx.x; item < a.n_work; item += gridDim.x) {
     const int2 wk = a.work[item];
@@ -339,11 +322,11 @@ __global__ __launch_bounds__(256, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
         xi_lds[c * xs + k] = src[c * a.spec_stride + k];
     __syncthreads();
 
-    for (int j = max(jb, i) + wv; j < je; j += 4) {
+    for (int j = max(jb, i) + wv; j < je; j += kWaves) {
       cf y[R];
       const float2* xj = a.spec + (long long)j * a.n_channels * a.spec_stride;
-      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, twl, prelane, wl, lane);
-      if constexpr (!FULL) apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
+      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
+      if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
       bool tp, tn;
       scan_wave<R>(y, lane, a.tie_tol, a.threshold, pos, neg, tp, tn);
@@ -374,9 +357,9 @@ __device__ double corr_f64(const float* __restrict__ ti, const float* __restrict
                            int P, int lag, int lane) {
   double s = 0.0;
   for (int n = lane; n < len; n += 64) {
-    int u = n - lag;
-    u %= P;
-    if (u < 0) u += P;
+    int u = n - lag;  // |lag| <= P/2 and n < P, so one wrap suffices
+    u += u < 0 ? P : 0;
+    u -= u >= P ? P : 0;
     if (u < len) s += (double)ti[n] * (double)tj[u];
   }
   return wave_sum_f64(s);
@@ -389,22 +372,18 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
   float2* twl = smem;
-  float2* xi_l = smem + TwLayout<R>::kSize;       // n_channels * (M+1)
-  float2* wl = xi_l + a.n_channels * (M + 1);
-  stage_twiddles<R>(twl, a.tw, lane, 64);
-  const cf prelane = f2c(a.pre_lane[lane]);
+  float2* wl = smem + TwLayout<R>::kInv;
+  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kInv, lane, 64);
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
   for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
     const int4 t = a.ties[rec];
     const int i = t.x, j = t.y, side = t.z;
-    const float2* si = a.spec + (long long)i * a.n_channels * a.spec_stride;
-    for (int c = 0; c < a.n_channels; ++c)
-      for (int k = lane; k <= M; k += 64) xi_l[c * (M + 1) + k] = si[c * a.spec_stride + k];
     wave_lds_fence();
     cf y[R];
-    pair_inverse<R>(y, xi_l, M + 1, a.spec + (long long)j * a.n_channels * a.spec_stride,
-                    a.spec_stride, a.n_channels, twl, prelane, wl, lane);
-    apply_window<R>(y, lane, a.lag_lo, a.lag_hi);
+    pair_inverse<R, true>(y, a.spec + (long long)i * a.n_channels * a.spec_stride, (int)a.spec_stride,
+                          a.spec + (long long)j * a.n_channels * a.spec_stride, a.spec_stride,
+                          a.n_channels, twl, wl, lane);
+    apply_window<R>(y, a.winmask, lane);
     const long long idx = packed_index(i, j, a.n_events);
     for (int sd = 0; sd < 2; ++sd) {
       if (!(side & (1 << sd))) continue;
@@ -471,22 +450,30 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
 template <int R>
 hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
   constexpr int M = 64 * R;
-  size_t lds = (size_t)(TwLayout<R>::kSize + kWaveLds + M) * sizeof(float2);
+  size_t lds = (size_t)(TwLayout<R>::kFft + kWaveLds + M) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_forward<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_forward<R>, dim3(a.n_rows), dim3(64), lds, st, a);
   return hipGetLastError();
 }
 
-template <int R, bool FULL>
-static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
-  size_t lds = pair_lds_bytes(R, a.n_channels);
-  static_assert(TwLayout<R>::kSize == 576 + 8 * R, "twiddle layout");
-  hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL>,
+template <int R, bool FULL, int W>
+static hipError_t launch_pairs_rfw(const PairArgs& a, int grid, hipStream_t st) {
+  size_t lds = pair_lds_bytes(R, a.n_channels, W);
+  static_assert(TwLayout<R>::kInv == 128 * R + 64, "twiddle layout");
+  hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL, W>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_pairs<R, FULL>), dim3(grid), dim3(256), lds, st, a);
+  hipLaunchKernelGGL((k_pairs<R, FULL, W>), dim3(grid), dim3(W * 64), lds, st, a);
   return hipGetLastError();
+}
+
+template <int R, bool FULL>
+static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
+  // 12 waves (3 per SIMD) when their LDS fits the 160 KiB of a CU, else 8
+  if (pair_lds_bytes(R, a.n_channels, 12) <= 160 * 1024)
+    return launch_pairs_rfw<R, FULL, 12>(a, grid, st);
+  return launch_pairs_rfw<R, FULL, 8>(a, grid, st);
 }
 
 template <int R>
@@ -496,7 +483,7 @@ hipError_t launch_pairs_r(bool full, const PairArgs& a, int grid, hipStream_t st
 
 template <int R>
 hipError_t launch_resolve_r(const ResolveArgs& a, int grid, hipStream_t st) {
-  size_t lds = (size_t)(TwLayout<R>::kSize + a.n_channels * (64 * R + 1) + kWaveLds) * sizeof(float2);
+  size_t lds = (size_t)(TwLayout<R>::kInv + kWaveLds) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_resolve<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resolve<R>, dim3(grid), dim3(64), lds, st, a);

@@ -29,6 +29,8 @@ def _check(gpu, ref, oracle_lags=None, name=""):
     vp, lp, vn, ln = gpu
     dp = np.abs(vp - ref["pos"]).max()
     dn = np.abs(vn - ref["neg"]).max()
+    print(f"{name}: max |dcoef| pos {dp:.3e} neg {dn:.3e}; FP64 gaps < 1e-5: "
+          f"{int((ref['gapp'] < 1e-5).sum())} pos, {int((ref['gapn'] < 1e-5).sum())} neg of {len(vp)}")
     assert dp <= COEF_TOL and dn <= COEF_TOL, f"{name}: max coef err pos {dp:.2e} neg {dn:.2e}"
     # GPU lags are the FP64 argmax unless the FP64 gap is tiny
     sure_p = ref["gapp"] >= GPU_EXACT_GAP
@@ -145,6 +147,8 @@ def test_extended_multichannel(oracle):
     vp, lp, vn, ln = [t.numpy() for t in outs]
     pi, pj = oracle.packed_pairs(n)
     ref = oracle.pairs_f64(ev, pi, pj, -P // 2 + 1, P // 2, P=P, circular=True)
+    print(f"multichannel: max |dcoef| pos {np.abs(vp - ref['pos']).max():.3e} "
+          f"neg {np.abs(vn - ref['neg']).max():.3e}")
     assert np.abs(vp - ref["pos"]).max() <= COEF_TOL
     assert np.abs(vn - ref["neg"]).max() <= COEF_TOL
     sure_p = ref["gap_pos"] >= GPU_EXACT_GAP
