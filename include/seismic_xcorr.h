@@ -37,7 +37,11 @@ enum {
 };
 
 enum {
-  SXC_FORM_FFT_F32 = 0,     /* batched R2C + fused conj-multiply/C2R/argmax, FP32 */
+  SXC_FORM_FFT_F32 = 0,     /* batched R2C + fused conj-multiply/C2R/argmax, FP32;
+                               circular lags mod padded_size */
+  SXC_FORM_LAG_BF16X3 = 1,  /* lag-window product on MFMA, bf16 hi/lo split (3 products),
+                               FP32 accumulation; linear lags, padded_size unused */
+  SXC_FORM_LAG_FP16 = 2,    /* lag-window product on MFMA, one FP16 product, FP32 accumulation */
 };
 
 typedef struct sxc_plan sxc_plan;
@@ -46,11 +50,12 @@ typedef struct {
   int n_events;      /* N >= 1 */
   int n_channels;    /* C in [1, 8]; channel correlations are averaged per lag */
   int trace_len;     /* L: samples entering the energy (and the mean) */
-  int padded_size;   /* P: FFT period, a power of two in [1024, 8192]; lags are circular mod P */
-  int lag_lo;        /* first scanned lag, > -P/2 */
-  int lag_hi;        /* last scanned lag, <= P/2, >= lag_lo */
+  int padded_size;   /* FFT form: period P, a power of two in [1024, 8192], lags circular mod P */
+  int lag_lo;        /* first scanned lag: FFT form > -P/2, lag forms > -L */
+  int lag_hi;        /* last scanned lag >= lag_lo: FFT form <= P/2, lag forms < L */
   float threshold;   /* pos kept if > threshold, neg if < -threshold */
-  float tie_tol;     /* near-tie band in coefficient units; <= 0 selects 1e-5 */
+  float tie_tol;     /* near-tie band in coefficient units; <= 0 selects the form's default
+                        (FFT 1e-5, bf16x3 2e-5, fp16 2e-4) */
   int demean;        /* 1: subtract the FP64 mean of the first L samples of each trace */
   int form;          /* SXC_FORM_* */
 } sxc_plan_desc;

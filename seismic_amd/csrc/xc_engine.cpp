@@ -19,6 +19,7 @@
 #include "../../include/seismic_xcorr.h"
 #include "xc_engine.h"
 #include "xc_kernels.h"
+#include "xc_lag.h"
 
 namespace sxc {
 
@@ -82,10 +83,21 @@ static int dev_alloc(T** p, size_t count, const char* what) {
   return SXC_OK;
 }
 
+static bool is_lag_form(int form) { return form == SXC_FORM_LAG_BF16X3 || form == SXC_FORM_LAG_FP16; }
+
 int validate_desc(const sxc_plan_desc& d) {
   const int P = d.padded_size;
   if (d.n_events < 1) { set_error("n_events must be >= 1 (got %d)", d.n_events); return SXC_ERR_INVALID; }
   if (d.n_channels < 1 || d.n_channels > 8) { set_error("n_channels must be in [1, 8] (got %d)", d.n_channels); return SXC_ERR_INVALID; }
+  if (is_lag_form(d.form)) {
+    if (d.trace_len < 1 || d.trace_len > (1 << 20)) { set_error("trace_len must be in [1, 2^20] (got %d)", d.trace_len); return SXC_ERR_INVALID; }
+    if (d.lag_lo > d.lag_hi || d.lag_lo <= -d.trace_len || d.lag_hi >= d.trace_len) {
+      set_error("lag window [%d, %d] must satisfy -L < lo <= hi < L (L=%d)", d.lag_lo, d.lag_hi, d.trace_len);
+      return SXC_ERR_INVALID;
+    }
+    if (!(d.threshold == d.threshold)) { set_error("threshold is NaN"); return SXC_ERR_INVALID; }
+    return SXC_OK;
+  }
   if (P < 1024 || P > 8192 || (P & (P - 1))) {
     set_error("padded_size must be a power of two in [1024, 8192] (got %d)", P);
     return SXC_ERR_INVALID;
@@ -125,6 +137,12 @@ struct sxc_plan {
   unsigned long long* winmask = nullptr;
   hipStream_t last_stream = nullptr;
   int device = 0;
+  // lag-window form
+  bool lag = false;
+  int l16 = 0, p0 = 0, ls = 0, lag_first = 0, n_blocks = 0, n_tiles = 0;
+  unsigned short *hi = nullptr, *lo = nullptr;
+  float* lag_traces = nullptr;
+  int2* tiles = nullptr;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -135,7 +153,7 @@ static void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
-                  p->fwd2, p->inv2, p->post, p->winmask};
+                  p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -204,6 +222,123 @@ static std::vector<int2> build_work(int n, int chunk) {
   return w;
 }
 
+// Lag-window form: zero-padded bf16 (hi, lo) or fp16 planes of every trace,
+// 32 x 32 pair tiles of the upper triangle, blocks of 8 lags.
+static int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
+  sxc_plan* p = new sxc_plan();
+  p->d = *desc;
+  p->lag = true;
+  const int L = desc->trace_len;
+  p->l16 = (L + 15) / 16 * 16;
+  // lag_first == 1 (mod 8) keeps every A and B window 16-byte aligned
+  int lf = desc->lag_lo;
+  while (((lf - 1) % 8 + 8) % 8 != 0) --lf;
+  p->lag_first = lf;
+  p->n_blocks = (desc->lag_hi - lf + 1 + 7) / 8;
+  const int last = lf + 8 * p->n_blocks - 1;
+  p->p0 = std::max(0, (last + 7 + 7) / 8 * 8);
+  p->ls = (std::max(p->p0 + p->l16, p->p0 + p->l16 - lf + 1) + 7) / 8 * 8;
+  p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
+  p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol
+                                   : (desc->form == SXC_FORM_LAG_FP16 ? 2e-4f : 2e-5f);
+  p->tie_cap = (int)std::min<long long>(p->n_pairs, 1LL << 24);
+  HIP_CHECK(hipGetDevice(&p->device));
+  const long long rows = (long long)desc->n_events * desc->n_channels;
+  auto fail = [&](int code) { plan_free(p); return code; };
+  int rc;
+  if ((rc = dev_alloc(&p->hi, (size_t)(rows * p->ls), "trace planes"))) return fail(rc);
+  if (desc->form == SXC_FORM_LAG_BF16X3 && (rc = dev_alloc(&p->lo, (size_t)(rows * p->ls), "trace planes")))
+    return fail(rc);
+  if ((rc = dev_alloc(&p->norms, (size_t)rows, "norms"))) return fail(rc);
+  if ((rc = dev_alloc(&p->lag_traces, (size_t)(rows * L), "FP32 traces"))) return fail(rc);
+  const int T = (desc->n_events + 31) / 32;
+  std::vector<int2> tiles;
+  for (int ti = 0; ti < T; ++ti)
+    for (int tj = ti; tj < T; ++tj) tiles.push_back(make_int2(ti, tj));
+  p->n_tiles = (int)tiles.size();
+  if ((rc = dev_alloc(&p->tiles, tiles.size(), "tile list"))) return fail(rc);
+  if (hipMemcpy(p->tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("copying the tile list failed");
+    return fail(SXC_ERR_HIP);
+  }
+  if ((rc = dev_alloc(&p->ties, (size_t)p->tie_cap * 2, "near-tie records"))) return fail(rc);
+  if ((rc = dev_alloc(&p->tie_count, 1, "near-tie counter"))) return fail(rc);
+  *out = p;
+  return SXC_OK;
+}
+
+static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                            float* vp, int* lp, float* vn, int* ln, hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
+  const int rows = d.n_events * d.n_channels;
+  hipEvent_t* ev = nullptr;
+  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
+  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
+  LagPrepArgs pa{};
+  pa.in = d_traces;
+  pa.in_stride = trace_stride;
+  pa.in_len = in_len;
+  pa.L = d.trace_len;
+  pa.n_rows = rows;
+  pa.n_channels = d.n_channels;
+  pa.demean = d.demean;
+  pa.fp16 = d.form == SXC_FORM_LAG_FP16;
+  pa.p0 = p->p0;
+  pa.ls = p->ls;
+  pa.hi = p->hi;
+  pa.lo = p->lo;
+  pa.norms = p->norms;
+  pa.traces_out = p->lag_traces;
+  HIP_CHECK(launch_lag_prep(pa, st));
+  HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
+  LagPairArgs ka{};
+  ka.hi = p->hi;
+  ka.lo = p->lo;
+  ka.ls = p->ls;
+  ka.n_events = d.n_events;
+  ka.n_channels = d.n_channels;
+  ka.l16 = p->l16;
+  ka.p0 = p->p0;
+  ka.lag_lo = d.lag_lo;
+  ka.lag_hi = d.lag_hi;
+  ka.lag_first = p->lag_first;
+  ka.n_blocks = p->n_blocks;
+  ka.tiles = p->tiles;
+  ka.n_tiles = p->n_tiles;
+  ka.threshold = d.threshold;
+  ka.tie_tol = p->tie_tol;
+  ka.vpos = vp;
+  ka.lpos = lp;
+  ka.vneg = vn;
+  ka.lneg = ln;
+  ka.ties = p->ties;
+  ka.tie_count = p->tie_count;
+  ka.tie_cap = p->tie_cap;
+  HIP_CHECK(launch_lag_pairs(d.form == SXC_FORM_LAG_BF16X3, ka, p->n_tiles, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
+  LagResolveArgs ra{};
+  ra.traces = p->lag_traces;
+  ra.tr_stride = d.trace_len;
+  ra.norms = p->norms;
+  ra.n_events = d.n_events;
+  ra.n_channels = d.n_channels;
+  ra.L = d.trace_len;
+  ra.lag_lo = d.lag_lo;
+  ra.lag_hi = d.lag_hi;
+  ra.threshold = d.threshold;
+  ra.ties = p->ties;
+  ra.tie_count = p->tie_count;
+  ra.tie_cap = p->tie_cap;
+  ra.vpos = vp;
+  ra.lpos = lp;
+  ra.vneg = vn;
+  ra.lneg = ln;
+  HIP_CHECK(launch_lag_resolve(ra, 2048, st));
+  if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
+  return SXC_OK;
+}
+
 extern "C" {
 
 const char* seismic_last_error(void) { return last_error(); }
@@ -214,6 +349,7 @@ int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   *out = nullptr;
   int rc = validate_desc(*desc);
   if (rc) return rc;
+  if (is_lag_form(desc->form)) return lag_plan_create(desc, out);
   sxc_plan* p = new sxc_plan();
   p->d = *desc;
   p->P = desc->padded_size;
@@ -257,12 +393,20 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
     set_error("null argument");
     return SXC_ERR_INVALID;
   }
-  if (in_len < 1 || in_len > p->P || trace_stride < in_len) {
+  if (!p->lag && (in_len < 1 || in_len > p->P || trace_stride < in_len)) {
     set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
     return SXC_ERR_INVALID;
   }
   hipStream_t st = (hipStream_t)stream;
   p->last_stream = st;
+  if (p->lag) {
+    if (in_len > trace_stride || in_len < 1) {
+      set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
+      return SXC_ERR_INVALID;
+    }
+    return lag_plan_execute(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,
+                            d_lags_neg, st);
+  }
   const sxc_plan_desc& d = p->d;
   const int rows = d.n_events * d.n_channels;
 

@@ -1,0 +1,66 @@
+// Argument blocks and launchers of the lag-window (MFMA) form.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sxc {
+
+struct LagPrepArgs {
+  const float* in;          // [n_rows][in_stride]
+  long long in_stride;
+  int in_len;               // samples present per row (zero beyond)
+  int L;                    // trace length used
+  int n_rows;
+  int n_channels;
+  int demean;
+  int fp16;                 // 1: FP16 plane only; 0: bf16 hi and lo planes
+  int p0;                   // zero samples before each trace in the planes
+  int ls;                   // plane row length (multiple of 8)
+  unsigned short* hi;       // [n_rows][ls]
+  unsigned short* lo;       // [n_rows][ls] or null
+  double* norms;            // [n_rows]
+  float* traces_out;        // [n_rows][L] FP32 copy (demeaned) for the FP64 resolver
+};
+
+struct LagPairArgs {
+  const unsigned short* hi;
+  const unsigned short* lo;
+  long long ls;
+  int n_events, n_channels;
+  int l16;                  // trace length rounded up to 16
+  int p0;
+  int lag_lo, lag_hi;       // scanned lags (inclusive)
+  int lag_first;            // first lag of block 0 (== 1 mod 8, <= lag_lo)
+  int n_blocks;             // blocks of 8 lags
+  const int2* tiles;        // (row tile, column tile), column >= row
+  int n_tiles;
+  float threshold, tie_tol;
+  float* vpos;
+  int* lpos;
+  float* vneg;
+  int* lneg;
+  int4* ties;               // 2 int4 per record: (i, j, sides, 0), (lp1, lp2, ln1, ln2)
+  unsigned* tie_count;
+  int tie_cap;
+};
+
+struct LagResolveArgs {
+  const float* traces;      // [n_events * n_channels][tr_stride]
+  long long tr_stride;
+  const double* norms;
+  int n_events, n_channels, L;
+  int lag_lo, lag_hi;
+  float threshold;
+  const int4* ties;
+  const unsigned* tie_count;
+  int tie_cap;
+  float* vpos;
+  int* lpos;
+  float* vneg;
+  int* lneg;
+};
+
+hipError_t launch_lag_prep(const LagPrepArgs& a, hipStream_t st);
+hipError_t launch_lag_pairs(bool bf16x3, const LagPairArgs& a, int grid, hipStream_t st);
+hipError_t launch_lag_resolve(const LagResolveArgs& a, int grid, hipStream_t st);
+
+}  // namespace sxc

@@ -1,0 +1,386 @@
+// Lag-window form on MFMA (gfx950): the correlation over a bounded window of
+// lags as a lag-blocked dense product.
+//
+// For pairs (i, j) in a 32 x 32 tile and a block of 8 consecutive lags
+// l = lb + d, the coefficient c_ij(l) = sum_c sum_n xh_ic[n] xh_jc[n - l]
+// (xh = x / (|x| sqrt(C)), linear correlation, zero outside [0, L)) is a
+// 32 x 32 x K product per lag with K running over all samples and channels:
+//   A (32 x 16) = row traces xh_i[n0 .. n0+16)       (held per K step)
+//   B (16 x 32) = column traces xh_j[n0 - l .. )      (a shifted window)
+// on v_mfma_f32_32x32x16_{bf16,f16}.  Each lane loads one 16-element window
+// of its column trace per K step and forms the 8 shifted fragments with
+// v_alignbyte (odd shifts) or plain register selection (even shifts).
+//
+// Precision forms
+//   bf16x3: xh = hi + lo with hi = bf16(xh), lo = bf16(xh - hi); the product
+//           is hi*hi + hi*lo + lo*hi with FP32 accumulation (3 MFMAs);
+//   fp16:   one FP16 product, FP32 accumulation.
+// FP64 is used for the per-trace energy and scale, and for the near-tie
+// re-evaluation (k_lag_resolve).  Scan rule as in the reference
+// (Correlation_in_OMP/correlation_c.cpp:102-136): pos = largest value above
+// 0 at its first lag, neg = smallest below 0, threshold to (0, 0).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "xc_lag.h"
+
+namespace sxc {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even FP32 -> bf16 bits (NaN kept quiet)
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// ----------------------------------------------------------------------------
+// prep: one workgroup per (event, channel) row
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lag_prep(LagPrepArgs a) {
+  const int row = blockIdx.x;
+  if (row >= a.n_rows) return;
+  const float* x = a.in + (long long)row * a.in_stride;
+  __shared__ double red[256];
+  double s1 = 0.0;
+  for (int n = threadIdx.x; n < a.L; n += 256) s1 += (double)(n < a.in_len ? x[n] : 0.f);
+  red[threadIdx.x] = s1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double mean = a.demean ? red[0] / (double)a.L : 0.0;
+  __syncthreads();
+  double s2 = 0.0;
+  for (int n = threadIdx.x; n < a.L; n += 256) {
+    const float v = n < a.in_len ? x[n] : 0.f;
+    const float u = a.demean ? (float)((double)v - mean) : v;
+    s2 += (double)u * (double)u;
+  }
+  red[threadIdx.x] = s2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double E = red[0];
+  if (threadIdx.x == 0) a.norms[row] = sqrt(E);
+  const double scale = E > 0.0 ? 1.0 / sqrt(E * (double)a.n_channels) : 0.0;
+  unsigned short* hi = a.hi + (long long)row * a.ls;
+  unsigned short* lo = a.lo ? a.lo + (long long)row * a.ls : nullptr;
+  float* tr = a.traces_out ? a.traces_out + (long long)row * a.L : nullptr;
+  for (int s = threadIdx.x; s < a.ls; s += 256) {
+    const int n = s - a.p0;
+    float v = 0.f;
+    if (n >= 0 && n < a.L && n < a.in_len) {
+      const float u = a.demean ? (float)((double)x[n] - mean) : x[n];
+      if (tr) tr[n] = u;
+      v = (float)((double)u * scale);
+    } else if (tr && n >= 0 && n < a.L) {
+      tr[n] = 0.f;
+    }
+    if (a.fp16) {
+      hi[s] = __builtin_bit_cast(unsigned short, (_Float16)v);
+    } else {
+      const unsigned short h = f32_to_bf16_rne(v);
+      hi[s] = h;
+      if (lo) lo[s] = f32_to_bf16_rne(v - bf16_to_f32(h));
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// tile kernel: one workgroup (4 waves) per 32 x 32 tile of pairs
+// ----------------------------------------------------------------------------
+// MFMA 32x32x16 operand layout: lane l supplies A[l % 32][8*(l/32) + e] and
+// B[8*(l/32) + e][l % 32], e < 8; result register r of lane l holds
+// D[8*(r/4) + 4*(l/32) + r%4][l % 32].
+__device__ __forceinline__ int mfma_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+struct TopState {
+  float p1, p2, n1, n2;  // best and second-best value, pos and neg sides
+  int lp1, lp2, ln1, ln2;
+};
+
+// 8 dwords (16 elements) -> the 4 dwords starting at element o (0 <= o <= 8)
+template <int O>
+__device__ __forceinline__ u32x4 window_frag(const unsigned (&w)[8]) {
+  u32x4 r;
+  if constexpr ((O & 1) == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = w[O / 2 + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_alignbyte(w[(O + 1) / 2 + k], w[(O - 1) / 2 + k], 2);
+  }
+  return r;
+}
+
+template <bool BF16X3>
+__device__ __forceinline__ void lag_block(f32x16 (&acc)[8], const unsigned short* __restrict__ ahi,
+                                          const unsigned short* __restrict__ alo,
+                                          const unsigned short* __restrict__ bhi,
+                                          const unsigned short* __restrict__ blo, long long ls,
+                                          int n_channels, int l16, int p0, int lb, int h) {
+#pragma unroll
+  for (int d = 0; d < 8; ++d) acc[d] = f32x16{};
+  for (int c = 0; c < n_channels; ++c) {
+    const unsigned short* ah = ahi + c * ls;
+    const unsigned short* al = BF16X3 ? alo + c * ls : nullptr;
+    const unsigned short* bh = bhi + c * ls;
+    const unsigned short* bl = BF16X3 ? blo + c * ls : nullptr;
+    for (int n0 = 0; n0 < l16; n0 += 16) {
+      const int ia = p0 + n0 + 8 * h;           // A: 8 samples, 16-byte aligned
+      const int ib = p0 + n0 + 8 * h - lb - 7;  // B window: 16 samples, 16-byte aligned
+      const u32x4 a_h = *reinterpret_cast<const u32x4*>(ah + ia);
+      u32x4 a_l;
+      if constexpr (BF16X3) a_l = *reinterpret_cast<const u32x4*>(al + ia);
+      unsigned wh[8], wl[8];
+      {
+        const u32x4 w0 = *reinterpret_cast<const u32x4*>(bh + ib);
+        const u32x4 w1 = *reinterpret_cast<const u32x4*>(bh + ib + 8);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { wh[k] = w0[k]; wh[4 + k] = w1[k]; }
+      }
+      if constexpr (BF16X3) {
+        const u32x4 w0 = *reinterpret_cast<const u32x4*>(bl + ib);
+        const u32x4 w1 = *reinterpret_cast<const u32x4*>(bl + ib + 8);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { wl[k] = w0[k]; wl[4 + k] = w1[k]; }
+      }
+      // lag lb + d uses window elements 7 - d .. 14 - d
+      auto step = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        const u32x4 b_h = window_frag<7 - d>(wh);
+        if constexpr (BF16X3) {
+          const u32x4 b_l = window_frag<7 - d>(wl);
+          const bf16x8 A_h = __builtin_bit_cast(bf16x8, a_h), A_l = __builtin_bit_cast(bf16x8, a_l);
+          const bf16x8 B_h = __builtin_bit_cast(bf16x8, b_h), B_l = __builtin_bit_cast(bf16x8, b_l);
+          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_l, B_h, acc[d], 0, 0, 0);
+          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_l, acc[d], 0, 0, 0);
+          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_h, acc[d], 0, 0, 0);
+        } else {
+          const f16x8 A_h = __builtin_bit_cast(f16x8, a_h), B_h = __builtin_bit_cast(f16x8, b_h);
+          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A_h, B_h, acc[d], 0, 0, 0);
+        }
+      };
+      step(std::integral_constant<int, 0>{});
+      step(std::integral_constant<int, 1>{});
+      step(std::integral_constant<int, 2>{});
+      step(std::integral_constant<int, 3>{});
+      step(std::integral_constant<int, 4>{});
+      step(std::integral_constant<int, 5>{});
+      step(std::integral_constant<int, 6>{});
+      step(std::integral_constant<int, 7>{});
+    }
+  }
+}
+
+template <bool BF16X3>
+__global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int h = lane >> 5;
+  __shared__ float sv[4][64][16][4];
+  __shared__ int sl[4][64][16][4];
+  for (int item = blockIdx.x; item < a.n_tiles; item += gridDim.x) {
+    const int2 t = a.tiles[item];
+    const int i0 = 32 * t.x, j0 = 32 * t.y;
+    // this lane's A row and B column (clamped: rows past N compute garbage that is never stored)
+    const int ia = min(i0 + (lane & 31), a.n_events - 1);
+    const int jb = min(j0 + (lane & 31), a.n_events - 1);
+    const long long rs = (long long)a.n_channels * a.ls;
+    const unsigned short* ahi = a.hi + ia * rs;
+    const unsigned short* alo = BF16X3 ? a.lo + ia * rs : nullptr;
+    const unsigned short* bhi = a.hi + jb * rs;
+    const unsigned short* blo = BF16X3 ? a.lo + jb * rs : nullptr;
+    float p1[16], p2[16], n1[16], n2[16];
+    int lp1[16], lp2[16], ln1[16], ln2[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      p1[r] = 0.f; p2[r] = -3.0e38f; n1[r] = 0.f; n2[r] = 3.0e38f;
+      lp1[r] = 0; lp2[r] = 0; ln1[r] = 0; ln2[r] = 0;
+    }
+    for (int b = wv; b < a.n_blocks; b += 4) {
+      const int lb = a.lag_first + 8 * b;
+      f32x16 acc[8];
+      lag_block<BF16X3>(acc, ahi, alo, bhi, blo, a.ls, a.n_channels, a.l16, a.p0, lb, h);
+      // top-2 per pair over the block's lags, in increasing lag order
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const int lag = lb + d;
+        const bool in = lag >= a.lag_lo && lag <= a.lag_hi;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float v = in ? acc[d][r] : 0.f;
+          const bool gp = v > p1[r];
+          const bool sp = !gp && v > p2[r];
+          p2[r] = gp ? p1[r] : (sp ? v : p2[r]);
+          lp2[r] = gp ? lp1[r] : (sp ? lag : lp2[r]);
+          p1[r] = gp ? v : p1[r];
+          lp1[r] = gp ? lag : lp1[r];
+          const bool gn = v < n1[r];
+          const bool sn = !gn && v < n2[r];
+          n2[r] = gn ? n1[r] : (sn ? v : n2[r]);
+          ln2[r] = gn ? ln1[r] : (sn ? lag : ln2[r]);
+          n1[r] = gn ? v : n1[r];
+          ln1[r] = gn ? lag : ln1[r];
+        }
+      }
+    }
+    // merge the 4 waves' states (lag blocks interleave, so ties go to the smaller lag)
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      sv[wv][lane][r][0] = p1[r]; sv[wv][lane][r][1] = p2[r];
+      sv[wv][lane][r][2] = n1[r]; sv[wv][lane][r][3] = n2[r];
+      sl[wv][lane][r][0] = lp1[r]; sl[wv][lane][r][1] = lp2[r];
+      sl[wv][lane][r][2] = ln1[r]; sl[wv][lane][r][3] = ln2[r];
+    }
+    __syncthreads();
+    if (wv == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        for (int w = 1; w < 4; ++w) {
+          const float q1 = sv[w][lane][r][0], q2 = sv[w][lane][r][1];
+          const int lq1 = sl[w][lane][r][0], lq2 = sl[w][lane][r][1];
+          // positive side; the starting 0 at lag 0 loses every tie
+          const bool takeq = q1 > p1[r] || (q1 == p1[r] && q1 > 0.f && (p1[r] <= 0.f || lq1 < lp1[r]));
+          float s1, s2; int ls1, ls2;
+          if (takeq) { s1 = q1; ls1 = lq1; s2 = p1[r]; ls2 = lp1[r]; }
+          else { s1 = p1[r]; ls1 = lp1[r]; s2 = q1; ls2 = lq1; }
+          const float o2 = fmaxf(p2[r], q2);
+          const int lo2 = p2[r] >= q2 ? lp2[r] : lq2;
+          if (o2 > s2) { s2 = o2; ls2 = lo2; }
+          p1[r] = s1; lp1[r] = ls1; p2[r] = s2; lp2[r] = ls2;
+          const float m1 = sv[w][lane][r][2], m2 = sv[w][lane][r][3];
+          const int lm1 = sl[w][lane][r][2], lm2 = sl[w][lane][r][3];
+          const bool takem = m1 < n1[r] || (m1 == n1[r] && m1 < 0.f && (n1[r] >= 0.f || lm1 < ln1[r]));
+          float t1, t2; int lt1, lt2;
+          if (takem) { t1 = m1; lt1 = lm1; t2 = n1[r]; lt2 = ln1[r]; }
+          else { t1 = n1[r]; lt1 = ln1[r]; t2 = m1; lt2 = lm1; }
+          const float u2 = fminf(n2[r], m2);
+          const int lu2 = n2[r] <= m2 ? ln2[r] : lm2;
+          if (u2 < t2) { t2 = u2; lt2 = lu2; }
+          n1[r] = t1; ln1[r] = lt1; n2[r] = t2; ln2[r] = lt2;
+        }
+        const int i = i0 + mfma_row(lane, r);
+        const int j = j0 + (lane & 31);
+        if (i < a.n_events && j < a.n_events && j >= i) {
+          const long long idx = (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
+          const bool kp = p1[r] > a.threshold && p1[r] > 0.f;
+          const bool kn = n1[r] < -a.threshold && n1[r] < 0.f;
+          a.vpos[idx] = kp ? p1[r] : 0.f;
+          a.lpos[idx] = kp ? lp1[r] : 0;
+          a.vneg[idx] = kn ? n1[r] : 0.f;
+          a.lneg[idx] = kn ? ln1[r] : 0;
+          const bool tp = kp && (p1[r] - p2[r] < a.tie_tol || p1[r] < a.tie_tol);
+          const bool tn = kn && (n2[r] - n1[r] < a.tie_tol || -n1[r] < a.tie_tol);
+          if (tp || tn) {
+            const unsigned slot = atomicAdd(a.tie_count, 1u);
+            if (slot < (unsigned)a.tie_cap) {
+              a.ties[2 * slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0), 0);
+              a.ties[2 * slot + 1] = make_int4(lp1[r], lp2[r], ln1[r], ln2[r]);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// FP64 resolver for the lag-window form: one wave per recorded pair
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, const float* __restrict__ tj,
+                                               int L, int lag, int lane) {
+  double s = 0.0;
+  const int n0 = lag > 0 ? lag : 0;
+  const int n1 = lag < 0 ? L + lag : L;
+  for (int n = n0 + lane; n < n1; n += 64) s += (double)ti[n] * (double)tj[n - lag];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+__global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
+  const int lane = threadIdx.x;
+  const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
+  for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
+    const int4 t0 = a.ties[2 * rec], t1 = a.ties[2 * rec + 1];
+    const int i = t0.x, j = t0.y, sides = t0.z;
+    const long long idx = (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!(sides & (1 << sd))) continue;
+      const int c0 = sd == 0 ? t1.x : t1.z, c1 = sd == 0 ? t1.y : t1.w;
+      const double sg = sd == 0 ? 1.0 : -1.0;
+      double best = 0.0;
+      int best_lag = 0;
+      bool have = false;
+      for (int q = 0; q < 2; ++q) {
+        const int lag = q == 0 ? c0 : c1;
+        if (lag < a.lag_lo || lag > a.lag_hi || (q == 1 && c1 == c0)) continue;
+        double acc = 0.0;
+        for (int c = 0; c < a.n_channels; ++c) {
+          const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
+          if (!(ni > 0.0 && nj > 0.0)) continue;
+          const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+          const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+          acc += lin_corr_f64(ti, tj, a.L, lag, lane) / (ni * nj * (double)a.n_channels);
+        }
+        acc *= sg;
+        if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
+          best = acc;
+          best_lag = lag;
+          have = true;
+        }
+      }
+      if (lane == 0) {
+        const float v = (float)best;
+        if (sd == 0) {
+          const bool keep = have && v > a.threshold;
+          a.vpos[idx] = keep ? v : 0.f;
+          a.lpos[idx] = keep ? best_lag : 0;
+        } else {
+          const bool keep = have && v > a.threshold;
+          a.vneg[idx] = keep ? -v : 0.f;
+          a.lneg[idx] = keep ? best_lag : 0;
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// launchers
+// ----------------------------------------------------------------------------
+hipError_t launch_lag_prep(const LagPrepArgs& a, hipStream_t st) {
+  if (a.n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lag_prep, dim3(a.n_rows), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_lag_pairs(bool bf16x3, const LagPairArgs& a, int grid, hipStream_t st) {
+  if (a.n_tiles <= 0 || grid <= 0) return hipSuccess;
+  if (bf16x3)
+    hipLaunchKernelGGL(k_lag_pairs<true>, dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_lag_pairs<false>, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_lag_resolve(const LagResolveArgs& a, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_lag_resolve, dim3(grid), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace sxc

@@ -10,6 +10,8 @@ import numpy as np
 from ._lib import SxcPlanDesc, SxcStats, SxcTiming, c_float_p, c_int_p, last_error, load_engine
 
 FORM_FFT_F32 = 0
+FORM_LAG_BF16X3 = 1
+FORM_LAG_FP16 = 2
 
 
 def _desc(n_events, n_channels, trace_len, padded_size, lag_lo, lag_hi, threshold, tie_tol,
@@ -103,7 +105,7 @@ class AllPairsPlan:
 
 
 def allpairs_host(traces, padded_size, lag_lo, lag_hi, trace_len=None, threshold=0.0,
-                  tie_tol=0.0, demean=False, in_len=None):
+                  tie_tol=0.0, demean=False, in_len=None, form=FORM_FFT_F32):
     """Host-memory convenience call.  traces: [N, C, stride] or [N, stride] float32."""
     lib = load_engine()
     tr = np.ascontiguousarray(traces, dtype=np.float32)
@@ -111,7 +113,7 @@ def allpairs_host(traces, padded_size, lag_lo, lag_hi, trace_len=None, threshold
         tr = tr[:, None, :]
     n, c, stride = tr.shape
     L = stride if trace_len is None else trace_len
-    d = _desc(n, c, L, padded_size, lag_lo, lag_hi, threshold, tie_tol, demean, FORM_FFT_F32)
+    d = _desc(n, c, L, padded_size, lag_lo, lag_hi, threshold, tie_tol, demean, form)
     npk = n_packed(n)
     vp = np.zeros(npk, np.float32); lp = np.zeros(npk, np.int32)
     vn = np.zeros(npk, np.float32); ln = np.zeros(npk, np.int32)

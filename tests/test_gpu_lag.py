@@ -1,0 +1,70 @@
+"""GPU parity of the lag-window MFMA forms against the FP64 direct oracle
+(linear correlation over [lag_lo, lag_hi]), through the C ABI.
+
+Bars: bf16x3 |dcoef| <= 1e-5 and lags equal to FP64 wherever the FP64 gap
+is at least 4e-5 (its near-tie band is 2e-5, settled in FP64);
+fp16 |dcoef| <= 2e-3, lags equal wherever the gap is at least 1e-3."""
+import numpy as np
+import pytest
+
+from seismic_amd import engine, synth
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {"bf16x3": (engine.FORM_LAG_BF16X3, 1e-5, 4e-5), "fp16": (engine.FORM_LAG_FP16, 2e-3, 1e-3)}
+
+
+def _run(ev, lo, hi, form, thr=0.0):
+    n, c, L = ev.shape
+    return engine.allpairs_host(ev, 0, lo, hi, threshold=thr, form=form)
+
+
+@pytest.mark.parametrize("name", ["bf16x3", "fp16"])
+@pytest.mark.parametrize("n,C,L,lo,hi,thr", [(64, 1, 1024, -100, 100, 0.0), (50, 1, 777, -37, 90, 0.1),
+                                           (40, 3, 2048, -512, 512, 0.0), (33, 2, 300, -299, 299, 0.2)])
+def test_lag_form_vs_f64(oracle, name, n, C, L, lo, hi, thr):
+    form, coef_tol, gap = FORMS[name]
+    ev, _ = synth.make_events(n, C, L, seed=n * 7 + L)
+    print(f"case lag {name} n={n} C={C} L={L} [{lo},{hi}] thr={thr}", flush=True)
+    vp, lp, vn, ln = _run(ev, lo, hi, form, thr)
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, lo, hi, circular=False)
+    rp = np.where(ref["pos"] > thr, ref["pos"], 0.0)
+    rn = np.where(ref["neg"] < -thr, ref["neg"], 0.0)
+    dp, dn = np.abs(vp - rp).max(), np.abs(vn - rn).max()
+    print(f"{name}: max |dcoef| pos {dp:.3e} neg {dn:.3e}")
+    assert dp <= coef_tol and dn <= coef_tol
+    kp = ref["pos"] > thr + 1e-3
+    kn = ref["neg"] < -thr - 1e-3
+    sure_p = kp & (ref["gap_pos"] >= gap)
+    sure_n = kn & (ref["gap_neg"] >= gap)
+    agree_p = (lp[kp] == ref["lag_pos"][kp]).mean() if kp.any() else 1.0
+    print(f"{name}: lag agreement pos {agree_p:.5f}")
+    assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
+    assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
+
+
+def test_lag_matches_fft_form():
+    """With P >= L + max|lag| the FFT form's circular lags equal linear lags:
+    the two forms must agree (values to 1e-5, lags off near ties)."""
+    n, C, L, P = 48, 3, 2048, 4096
+    ev, _ = synth.make_events(n, C, L, seed=5)
+    a = engine.allpairs_host(ev, P, -512, 512)
+    b = engine.allpairs_host(ev, 0, -512, 512, form=engine.FORM_LAG_BF16X3)
+    assert np.abs(a[0] - b[0]).max() < 2e-5 and np.abs(a[2] - b[2]).max() < 2e-5
+    assert (a[1] == b[1]).mean() > 0.999 and (a[3] == b[3]).mean() > 0.999
+
+
+def test_lag_exact_shift_duplicates():
+    """A zero-filled time-shifted copy peaks at exactly its shift with value
+    1 (up to precision), in both precision forms."""
+    rng = np.random.default_rng(3)
+    L = 1024
+    base = np.zeros((1, L), np.float32)
+    base[0, 300:700] = rng.standard_normal(400)
+    ev = np.stack([base, np.roll(base, 41, axis=1), np.roll(base, -77, axis=1)])
+    for form, tol in ((engine.FORM_LAG_BF16X3, 1e-5), (engine.FORM_LAG_FP16, 2e-3)):
+        vp, lp, vn, ln = engine.allpairs_host(ev, 0, -200, 200, form=form)
+        # packed (0,1)=1, (0,2)=2, (1,2)=4
+        assert lp[1] == -41 and lp[2] == 77 and lp[4] == 118
+        assert abs(vp[1] - 1) < tol and abs(vp[2] - 1) < tol and abs(vp[0] - 1) < tol
