@@ -27,7 +27,9 @@ sys.path.insert(0, HERE)
 
 METRIC = "pairs/sec N=4096 L=2048: FP32-FFT vs MFMA-bf16x3; serve p50/p99 ms"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+LAG_WINDOW = 512  # the lag-window forms are compared at +-512
 
 
 def log(*a):
@@ -39,6 +41,73 @@ def fft_flops_per_pair(C: int, P: int) -> float:
     (8 flops per bin) plus one stacked C2R (2.5 P log2 P)."""
     B = P // 2 + 1
     return C * 8.0 * B + 2.5 * P * math.log2(P)
+
+
+def time_plan(plan, tr, outs, stream, steps, warmup):
+    from seismic_amd.device import synchronize
+    for _ in range(warmup):
+        plan.execute(tr, out=outs, stream=stream)
+    stream.synchronize()
+    plan.enable_timing(True)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(tr, out=outs, stream=stream)
+    stream.synchronize()
+    synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, plan.timing(), plan.stats()
+
+
+def lag_forms(ev, tr, stream, steps, warmup):
+    """MFMA lag-window forms and the FFT form on the same +-LAG_WINDOW window."""
+    from seismic_amd import engine
+    N, C, L = ev.shape
+    w = LAG_WINDOW
+    out = {}
+    for name, form in (("bf16x3", engine.FORM_LAG_BF16X3), ("fp16", engine.FORM_LAG_FP16)):
+        plan = engine.AllPairsPlan(N, C, L, 0, -w, w, threshold=0.0, form=form)
+        outs = plan.alloc_outputs()
+        ms, tm, st = time_plan(plan, tr, outs, stream, steps, warmup)
+        k = 3 if name == "bf16x3" else 1
+        T = (N + 31) // 32
+        n_tiles = T * (T + 1) // 2
+        lag_first = -w
+        while (lag_first - 1) % 8:
+            lag_first -= 1
+        n_lags_issued = 8 * ((w - lag_first + 1 + 7) // 8)
+        l16 = (L + 15) // 16 * 16
+        issued = n_tiles * 1024 * k * 2.0 * n_lags_issued * l16 * C
+        useful = N * (N - 1) / 2 * 2.0 * ((2 * w + 1) * L - w * (w + 1)) * C
+        kern_s = tm["ms_pairs"] / 1e3
+        out[name] = {
+            "value": round(N * (N - 1) / 2 / (ms / 1e3), 1), "unit": "pairs/s",
+            "ms_per_step": round(ms, 3), "lags": [-w, w],
+            "stages_ms": {k2: round(v, 4) for k2, v in tm.items() if k2 != "executes"},
+            "near_ties_per_step": st["near_ties"],
+            "roofline": {"bound": "mfma", "achieved": round(issued / kern_s / 1e12, 2),
+                         "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(issued / kern_s / 1e12 / PEAK_BF16_TFLOPS, 4),
+                         "issued_flops_per_launch": issued,
+                         "useful_tflops": round(useful / kern_s / 1e12, 2)},
+            "_outs": [o.numpy() for o in outs],
+        }
+        plan.close()
+    plan = engine.AllPairsPlan(N, C, L, 4096, -w, w, threshold=0.0)
+    outs = plan.alloc_outputs()
+    ms, tm, st = time_plan(plan, tr, outs, stream, steps, warmup)
+    out["fft_same_window"] = {"value": round(N * (N - 1) / 2 / (ms / 1e3), 1), "unit": "pairs/s",
+                              "ms_per_step": round(ms, 3), "_outs": [o.numpy() for o in outs]}
+    plan.close()
+    # agreement with the FFT form on the same window (both against each other)
+    ref = out["fft_same_window"].pop("_outs")
+    for name in ("bf16x3", "fp16"):
+        o = out[name].pop("_outs")
+        out[name]["max_abs_dcoef_vs_fft_f32"] = float(max(np.abs(o[0] - ref[0]).max(),
+                                                          np.abs(o[2] - ref[2]).max()))
+        out[name]["lag_agreement_vs_fft_f32"] = float(((o[1] == ref[1]) & (o[3] == ref[3])).mean())
+    out["ratio_bf16x3_over_fft"] = round(out["bf16x3"]["value"] / out["fft_same_window"]["value"], 4)
+    return out
 
 
 def cpu_baseline(ev: np.ndarray, P: int, S: int, sample_events: int, threads: int):
@@ -66,6 +135,8 @@ def main():
     ap.add_argument("--n-events", type=int, default=4096)
     ap.add_argument("--cpu-sample-events", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-lag-forms", action="store_true", help="skip the MFMA lag-window forms")
+    ap.add_argument("--lag-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (scales PMC byte counters)")
     args = ap.parse_args()
@@ -98,23 +169,7 @@ def main():
         synchronize()
         log("bench: copy calibration 3 x 256 MiB done")
 
-    for _ in range(args.warmup):
-        plan.execute(tr, out=outs, stream=stream)
-    stream.synchronize()
-    st = plan.stats()
-    log(f"bench: warmup done, near ties per step {st['near_ties']}")
-
-    plan.enable_timing(True)
-    synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute(tr, out=outs, stream=stream)
-    stream.synchronize()
-    synchronize()
-    t1 = time.perf_counter()
-    tm = plan.timing()
-    st = plan.stats()
-    ms = (t1 - t0) * 1e3 / args.steps
+    ms, tm, st = time_plan(plan, tr, outs, stream, args.steps, args.warmup)
     unique = N * (N - 1) / 2
     packed = N * (N + 1) / 2
     value = unique / (ms / 1e3)
@@ -135,6 +190,13 @@ def main():
                 "flops_per_pair": fpp, "pairs_per_launch": int(packed),
                 "kernel_ms": round(tm["ms_pairs"], 4)}
 
+    lag = None
+    if not args.no_lag_forms:
+        log(f"bench: lag-window forms at +-{LAG_WINDOW}")
+        lag = lag_forms(ev, tr, stream, args.lag_steps, 1)
+        log(f"bench: bf16x3 {lag['bf16x3']['ms_per_step']} ms, fp16 {lag['fp16']['ms_per_step']} ms, "
+            f"fft {lag['fft_same_window']['ms_per_step']} ms")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -154,7 +216,7 @@ def main():
         "near_ties_per_step": st["near_ties"],
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "mfma_bf16x3": None,
+        "mfma_bf16x3": lag,
         "serving": None,
     }
     print(json.dumps(line), flush=True)

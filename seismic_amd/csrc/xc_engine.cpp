@@ -230,14 +230,16 @@ static int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   p->lag = true;
   const int L = desc->trace_len;
   p->l16 = (L + 15) / 16 * 16;
-  // lag_first == 1 (mod 8) keeps every A and B window 16-byte aligned
+  // lag_first == 1 (mod 8) keeps every A and B window 16-byte aligned:
+  // the B window of block lb starts at p0 + n0 + 8h - lb - (kLagBlock - 1).
   int lf = desc->lag_lo;
   while (((lf - 1) % 8 + 8) % 8 != 0) --lf;
   p->lag_first = lf;
-  p->n_blocks = (desc->lag_hi - lf + 1 + 7) / 8;
-  const int last = lf + 8 * p->n_blocks - 1;
-  p->p0 = std::max(0, (last + 7 + 7) / 8 * 8);
-  p->ls = (std::max(p->p0 + p->l16, p->p0 + p->l16 - lf + 1) + 7) / 8 * 8;
+  p->n_blocks = (desc->lag_hi - lf + 1 + kLagBlock - 1) / kLagBlock;
+  const int last = lf + kLagBlock * p->n_blocks - 1;
+  // lowest B index: p0 - last >= 0; highest: p0 + l16 - lf + 8 (24-element windows)
+  p->p0 = std::max(0, (last + 7) / 8 * 8);
+  p->ls = (std::max(p->p0 + p->l16, p->p0 + p->l16 - lf + 9) + 7) / 8 * 8;
   p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
   p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol
                                    : (desc->form == SXC_FORM_LAG_FP16 ? 2e-4f : 2e-5f);

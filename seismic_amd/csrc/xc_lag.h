@@ -4,6 +4,8 @@
 
 namespace sxc {
 
+constexpr int kLagBlock = 16;  // lags per block (one accumulator tile each)
+
 struct LagPrepArgs {
   const float* in;          // [n_rows][in_stride]
   long long in_stride;
@@ -30,7 +32,7 @@ struct LagPairArgs {
   int p0;
   int lag_lo, lag_hi;       // scanned lags (inclusive)
   int lag_first;            // first lag of block 0 (== 1 mod 8, <= lag_lo)
-  int n_blocks;             // blocks of 8 lags
+  int n_blocks;             // blocks of kLagBlock lags
   const int2* tiles;        // (row tile, column tile), column >= row
   int n_tiles;
   float threshold, tie_tol;

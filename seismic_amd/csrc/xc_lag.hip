@@ -1,15 +1,16 @@
 // Lag-window form on MFMA (gfx950): the correlation over a bounded window of
 // lags as a lag-blocked dense product.
 //
-// For pairs (i, j) in a 32 x 32 tile and a block of 8 consecutive lags
+// For pairs (i, j) in a 32 x 32 tile and a block of kLagBlock (16) consecutive lags
 // l = lb + d, the coefficient c_ij(l) = sum_c sum_n xh_ic[n] xh_jc[n - l]
 // (xh = x / (|x| sqrt(C)), linear correlation, zero outside [0, L)) is a
 // 32 x 32 x K product per lag with K running over all samples and channels:
 //   A (32 x 16) = row traces xh_i[n0 .. n0+16)       (held per K step)
 //   B (16 x 32) = column traces xh_j[n0 - l .. )      (a shifted window)
-// on v_mfma_f32_32x32x16_{bf16,f16}.  Each lane loads one 16-element window
-// of its column trace per K step and forms the 8 shifted fragments with
-// v_alignbyte (odd shifts) or plain register selection (even shifts).
+// on v_mfma_f32_32x32x16_{bf16,f16}.  Each lane loads one 24-element window
+// of its column trace per K step and forms the 16 shifted fragments with
+// v_alignbyte (odd shifts) or plain register selection (even shifts).  The
+// 16 lag accumulators (32 x 32 FP32 each) live in the accumulation registers.
 //
 // Precision forms
 //   bf16x3: xh = hi + lo with hi = bf16(xh), lo = bf16(xh - hi); the product
@@ -106,96 +107,122 @@ __global__ __launch_bounds__(256) void k_lag_prep(LagPrepArgs a) {
 // D[8*(r/4) + 4*(l/32) + r%4][l % 32].
 __device__ __forceinline__ int mfma_row(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-struct TopState {
-  float p1, p2, n1, n2;  // best and second-best value, pos and neg sides
-  int lp1, lp2, ln1, ln2;
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor_lag(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor_lag<B + 1, E>(f);
+  }
+}
+
+// One K step's operands for this lane: A (row) 8 samples and the B (column)
+// window of 24 samples, hi and (bf16x3) lo planes.
+struct StepData {
+  u32x4 ah, al;
+  u32x4 bh[3], bl[3];
 };
 
-// 8 dwords (16 elements) -> the 4 dwords starting at element o (0 <= o <= 8)
+template <bool BF16X3>
+__device__ __forceinline__ void load_step(StepData& s, const unsigned short* __restrict__ ahi,
+                                          const unsigned short* __restrict__ alo,
+                                          const unsigned short* __restrict__ bhi,
+                                          const unsigned short* __restrict__ blo, long long ls,
+                                          int steps_per_channel, int step, int p0, int lb, int h) {
+  const int c = step / steps_per_channel;
+  const int n0 = 16 * (step - c * steps_per_channel);
+  const long long ia = c * ls + p0 + n0 + 8 * h;            // A: 8 samples, 16-byte aligned
+  const long long ib = c * ls + p0 + n0 + 8 * h - lb - 15;  // B: 24 samples, 16-byte aligned
+  s.ah = *reinterpret_cast<const u32x4*>(ahi + ia);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) s.bh[q] = *reinterpret_cast<const u32x4*>(bhi + ib + 8 * q);
+  if constexpr (BF16X3) {
+    s.al = *reinterpret_cast<const u32x4*>(alo + ia);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s.bl[q] = *reinterpret_cast<const u32x4*>(blo + ib + 8 * q);
+  }
+}
+
+// 12 dwords (24 elements) -> the 4 dwords starting at element O (0 <= O <= 16)
 template <int O>
-__device__ __forceinline__ u32x4 window_frag(const unsigned (&w)[8]) {
+__device__ __forceinline__ u32x4 window_frag3(const u32x4 (&w)[3]) {
+  auto word = [&](int k) { return w[k >> 2][k & 3]; };
   u32x4 r;
   if constexpr ((O & 1) == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) r[k] = w[O / 2 + k];
+    for (int k = 0; k < 4; ++k) r[k] = word(O / 2 + k);
   } else {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_alignbyte(w[(O + 1) / 2 + k], w[(O - 1) / 2 + k], 2);
+    for (int k = 0; k < 4; ++k) r[k] = __builtin_amdgcn_alignbyte(word((O + 1) / 2 + k), word((O - 1) / 2 + k), 2);
   }
   return r;
 }
 
 template <bool BF16X3>
-__device__ __forceinline__ void lag_block(f32x16 (&acc)[8], const unsigned short* __restrict__ ahi,
+__device__ __forceinline__ void mfma_step(f32x16 (&acc)[kLagBlock], const StepData& s) {
+  // lag lb + d uses window elements 15 - d .. 22 - d
+  auto step = [&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    const u32x4 b_h = window_frag3<15 - d>(s.bh);
+    if constexpr (BF16X3) {
+      const u32x4 b_l = window_frag3<15 - d>(s.bl);
+      const bf16x8 A_h = __builtin_bit_cast(bf16x8, s.ah), A_l = __builtin_bit_cast(bf16x8, s.al);
+      const bf16x8 B_h = __builtin_bit_cast(bf16x8, b_h), B_l = __builtin_bit_cast(bf16x8, b_l);
+      acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_l, B_h, acc[d], 0, 0, 0);
+      acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_l, acc[d], 0, 0, 0);
+      acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_h, acc[d], 0, 0, 0);
+    } else {
+      const f16x8 A_h = __builtin_bit_cast(f16x8, s.ah), B_h = __builtin_bit_cast(f16x8, b_h);
+      acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A_h, B_h, acc[d], 0, 0, 0);
+    }
+  };
+  sfor_lag<0, kLagBlock>(step);
+}
+
+// All K steps (channels x samples) of one block of kLagBlock lags.  Operands
+// are loaded kDepth steps ahead (a ring of kDepth buffers) so one wave per
+// SIMD keeps the MFMA pipe fed: 2 steps for bf16x3 (48 MFMAs per step), 4 for
+// fp16 (16 MFMAs per step, fewer registers per buffer).
+template <bool BF16X3>
+__device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsigned short* __restrict__ ahi,
                                           const unsigned short* __restrict__ alo,
                                           const unsigned short* __restrict__ bhi,
                                           const unsigned short* __restrict__ blo, long long ls,
                                           int n_channels, int l16, int p0, int lb, int h) {
+  constexpr int kDepth = BF16X3 ? 2 : 4;
 #pragma unroll
-  for (int d = 0; d < 8; ++d) acc[d] = f32x16{};
-  for (int c = 0; c < n_channels; ++c) {
-    const unsigned short* ah = ahi + c * ls;
-    const unsigned short* al = BF16X3 ? alo + c * ls : nullptr;
-    const unsigned short* bh = bhi + c * ls;
-    const unsigned short* bl = BF16X3 ? blo + c * ls : nullptr;
-    for (int n0 = 0; n0 < l16; n0 += 16) {
-      const int ia = p0 + n0 + 8 * h;           // A: 8 samples, 16-byte aligned
-      const int ib = p0 + n0 + 8 * h - lb - 7;  // B window: 16 samples, 16-byte aligned
-      const u32x4 a_h = *reinterpret_cast<const u32x4*>(ah + ia);
-      u32x4 a_l;
-      if constexpr (BF16X3) a_l = *reinterpret_cast<const u32x4*>(al + ia);
-      unsigned wh[8], wl[8];
-      {
-        const u32x4 w0 = *reinterpret_cast<const u32x4*>(bh + ib);
-        const u32x4 w1 = *reinterpret_cast<const u32x4*>(bh + ib + 8);
+  for (int d = 0; d < kLagBlock; ++d) acc[d] = f32x16{};
+  const int spc = l16 / 16;
+  const int total = n_channels * spc;  // >= 1
+  StepData buf[kDepth];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { wh[k] = w0[k]; wh[4 + k] = w1[k]; }
-      }
-      if constexpr (BF16X3) {
-        const u32x4 w0 = *reinterpret_cast<const u32x4*>(bl + ib);
-        const u32x4 w1 = *reinterpret_cast<const u32x4*>(bl + ib + 8);
+  for (int q = 0; q < kDepth; ++q)
+    load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ls, spc, min(q, total - 1), p0, lb, h);
+  for (int st = 0; st < total; st += kDepth) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { wl[k] = w0[k]; wl[4 + k] = w1[k]; }
-      }
-      // lag lb + d uses window elements 7 - d .. 14 - d
-      auto step = [&](auto dc) {
-        constexpr int d = decltype(dc)::value;
-        const u32x4 b_h = window_frag<7 - d>(wh);
-        if constexpr (BF16X3) {
-          const u32x4 b_l = window_frag<7 - d>(wl);
-          const bf16x8 A_h = __builtin_bit_cast(bf16x8, a_h), A_l = __builtin_bit_cast(bf16x8, a_l);
-          const bf16x8 B_h = __builtin_bit_cast(bf16x8, b_h), B_l = __builtin_bit_cast(bf16x8, b_l);
-          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_l, B_h, acc[d], 0, 0, 0);
-          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_l, acc[d], 0, 0, 0);
-          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A_h, B_h, acc[d], 0, 0, 0);
-        } else {
-          const f16x8 A_h = __builtin_bit_cast(f16x8, a_h), B_h = __builtin_bit_cast(f16x8, b_h);
-          acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A_h, B_h, acc[d], 0, 0, 0);
-        }
-      };
-      step(std::integral_constant<int, 0>{});
-      step(std::integral_constant<int, 1>{});
-      step(std::integral_constant<int, 2>{});
-      step(std::integral_constant<int, 3>{});
-      step(std::integral_constant<int, 4>{});
-      step(std::integral_constant<int, 5>{});
-      step(std::integral_constant<int, 6>{});
-      step(std::integral_constant<int, 7>{});
+    for (int q = 0; q < kDepth; ++q) {
+      if (st + q < total) mfma_step<BF16X3>(acc, buf[q]);
+      // refill (clamped at the end: a repeated in-bounds load is harmless)
+      load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ls, spc, min(st + q + kDepth, total - 1), p0, lb, h);
     }
   }
 }
+
+// Running per-pair state of one wave, in LDS: [value/lag][pos1, pos2, neg1, neg2]
+struct LagState {
+  float v[4][64][16][4];
+  int l[4][64][16][4];
+};
 
 template <bool BF16X3>
 __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int h = lane >> 5;
-  __shared__ float sv[4][64][16][4];
-  __shared__ int sl[4][64][16][4];
+  __shared__ LagState st;
   for (int item = blockIdx.x; item < a.n_tiles; item += gridDim.x) {
     const int2 t = a.tiles[item];
     const int i0 = 32 * t.x, j0 = 32 * t.y;
-    // this lane's A row and B column (clamped: rows past N compute garbage that is never stored)
+    // this lane's A row and B column (clamped: rows past N compute values that are never stored)
     const int ia = min(i0 + (lane & 31), a.n_events - 1);
     const int jb = min(j0 + (lane & 31), a.n_events - 1);
     const long long rs = (long long)a.n_channels * a.ls;
@@ -203,93 +230,99 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     const unsigned short* alo = BF16X3 ? a.lo + ia * rs : nullptr;
     const unsigned short* bhi = a.hi + jb * rs;
     const unsigned short* blo = BF16X3 ? a.lo + jb * rs : nullptr;
-    float p1[16], p2[16], n1[16], n2[16];
-    int lp1[16], lp2[16], ln1[16], ln2[16];
+    __syncthreads();  // the previous tile's merge has read the state
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      p1[r] = 0.f; p2[r] = -3.0e38f; n1[r] = 0.f; n2[r] = 3.0e38f;
-      lp1[r] = 0; lp2[r] = 0; ln1[r] = 0; ln2[r] = 0;
+      st.v[wv][lane][r][0] = 0.f; st.v[wv][lane][r][1] = -3.0e38f;
+      st.v[wv][lane][r][2] = 0.f; st.v[wv][lane][r][3] = 3.0e38f;
+      st.l[wv][lane][r][0] = 0; st.l[wv][lane][r][1] = 0; st.l[wv][lane][r][2] = 0; st.l[wv][lane][r][3] = 0;
     }
     for (int b = wv; b < a.n_blocks; b += 4) {
-      const int lb = a.lag_first + 8 * b;
-      f32x16 acc[8];
+      const int lb = a.lag_first + kLagBlock * b;
+      f32x16 acc[kLagBlock];
       lag_block<BF16X3>(acc, ahi, alo, bhi, blo, a.ls, a.n_channels, a.l16, a.p0, lb, h);
       // top-2 per pair over the block's lags, in increasing lag order
 #pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const int lag = lb + d;
-        const bool in = lag >= a.lag_lo && lag <= a.lag_hi;
+      for (int r = 0; r < 16; ++r) {
+        float p1 = st.v[wv][lane][r][0], p2 = st.v[wv][lane][r][1];
+        float n1 = st.v[wv][lane][r][2], n2 = st.v[wv][lane][r][3];
+        int lp1 = st.l[wv][lane][r][0], lp2 = st.l[wv][lane][r][1];
+        int ln1 = st.l[wv][lane][r][2], ln2 = st.l[wv][lane][r][3];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
+        for (int d = 0; d < kLagBlock; ++d) {
+          const int lag = lb + d;
+          const bool in = lag >= a.lag_lo && lag <= a.lag_hi;
           const float v = in ? acc[d][r] : 0.f;
-          const bool gp = v > p1[r];
-          const bool sp = !gp && v > p2[r];
-          p2[r] = gp ? p1[r] : (sp ? v : p2[r]);
-          lp2[r] = gp ? lp1[r] : (sp ? lag : lp2[r]);
-          p1[r] = gp ? v : p1[r];
-          lp1[r] = gp ? lag : lp1[r];
-          const bool gn = v < n1[r];
-          const bool sn = !gn && v < n2[r];
-          n2[r] = gn ? n1[r] : (sn ? v : n2[r]);
-          ln2[r] = gn ? ln1[r] : (sn ? lag : ln2[r]);
-          n1[r] = gn ? v : n1[r];
-          ln1[r] = gn ? lag : ln1[r];
+          const bool gp = v > p1;
+          const bool sp = !gp && v > p2;
+          p2 = gp ? p1 : (sp ? v : p2);
+          lp2 = gp ? lp1 : (sp ? lag : lp2);
+          p1 = gp ? v : p1;
+          lp1 = gp ? lag : lp1;
+          const bool gn = v < n1;
+          const bool sn = !gn && v < n2;
+          n2 = gn ? n1 : (sn ? v : n2);
+          ln2 = gn ? ln1 : (sn ? lag : ln2);
+          n1 = gn ? v : n1;
+          ln1 = gn ? lag : ln1;
         }
+        st.v[wv][lane][r][0] = p1; st.v[wv][lane][r][1] = p2;
+        st.v[wv][lane][r][2] = n1; st.v[wv][lane][r][3] = n2;
+        st.l[wv][lane][r][0] = lp1; st.l[wv][lane][r][1] = lp2;
+        st.l[wv][lane][r][2] = ln1; st.l[wv][lane][r][3] = ln2;
       }
     }
     // merge the 4 waves' states (lag blocks interleave, so ties go to the smaller lag)
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      sv[wv][lane][r][0] = p1[r]; sv[wv][lane][r][1] = p2[r];
-      sv[wv][lane][r][2] = n1[r]; sv[wv][lane][r][3] = n2[r];
-      sl[wv][lane][r][0] = lp1[r]; sl[wv][lane][r][1] = lp2[r];
-      sl[wv][lane][r][2] = ln1[r]; sl[wv][lane][r][3] = ln2[r];
-    }
-    __syncthreads();
     if (wv == 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
+        float p1 = st.v[0][lane][r][0], p2 = st.v[0][lane][r][1];
+        float n1 = st.v[0][lane][r][2], n2 = st.v[0][lane][r][3];
+        int lp1 = st.l[0][lane][r][0], lp2 = st.l[0][lane][r][1];
+        int ln1 = st.l[0][lane][r][2], ln2 = st.l[0][lane][r][3];
         for (int w = 1; w < 4; ++w) {
-          const float q1 = sv[w][lane][r][0], q2 = sv[w][lane][r][1];
-          const int lq1 = sl[w][lane][r][0], lq2 = sl[w][lane][r][1];
+          const float q1 = st.v[w][lane][r][0], q2 = st.v[w][lane][r][1];
+          const int lq1 = st.l[w][lane][r][0], lq2 = st.l[w][lane][r][1];
           // positive side; the starting 0 at lag 0 loses every tie
-          const bool takeq = q1 > p1[r] || (q1 == p1[r] && q1 > 0.f && (p1[r] <= 0.f || lq1 < lp1[r]));
-          float s1, s2; int ls1, ls2;
-          if (takeq) { s1 = q1; ls1 = lq1; s2 = p1[r]; ls2 = lp1[r]; }
-          else { s1 = p1[r]; ls1 = lp1[r]; s2 = q1; ls2 = lq1; }
-          const float o2 = fmaxf(p2[r], q2);
-          const int lo2 = p2[r] >= q2 ? lp2[r] : lq2;
+          const bool takeq = q1 > p1 || (q1 == p1 && q1 > 0.f && lq1 < lp1);
+          float s1, s2;
+          int ls1, ls2;
+          if (takeq) { s1 = q1; ls1 = lq1; s2 = p1; ls2 = lp1; }
+          else { s1 = p1; ls1 = lp1; s2 = q1; ls2 = lq1; }
+          const float o2 = fmaxf(p2, q2);
+          const int lo2 = p2 >= q2 ? lp2 : lq2;
           if (o2 > s2) { s2 = o2; ls2 = lo2; }
-          p1[r] = s1; lp1[r] = ls1; p2[r] = s2; lp2[r] = ls2;
-          const float m1 = sv[w][lane][r][2], m2 = sv[w][lane][r][3];
-          const int lm1 = sl[w][lane][r][2], lm2 = sl[w][lane][r][3];
-          const bool takem = m1 < n1[r] || (m1 == n1[r] && m1 < 0.f && (n1[r] >= 0.f || lm1 < ln1[r]));
-          float t1, t2; int lt1, lt2;
-          if (takem) { t1 = m1; lt1 = lm1; t2 = n1[r]; lt2 = ln1[r]; }
-          else { t1 = n1[r]; lt1 = ln1[r]; t2 = m1; lt2 = lm1; }
-          const float u2 = fminf(n2[r], m2);
-          const int lu2 = n2[r] <= m2 ? ln2[r] : lm2;
+          p1 = s1; lp1 = ls1; p2 = s2; lp2 = ls2;
+          const float m1 = st.v[w][lane][r][2], m2 = st.v[w][lane][r][3];
+          const int lm1 = st.l[w][lane][r][2], lm2 = st.l[w][lane][r][3];
+          const bool takem = m1 < n1 || (m1 == n1 && m1 < 0.f && lm1 < ln1);
+          float t1, t2;
+          int lt1, lt2;
+          if (takem) { t1 = m1; lt1 = lm1; t2 = n1; lt2 = ln1; }
+          else { t1 = n1; lt1 = ln1; t2 = m1; lt2 = lm1; }
+          const float u2 = fminf(n2, m2);
+          const int lu2 = n2 <= m2 ? ln2 : lm2;
           if (u2 < t2) { t2 = u2; lt2 = lu2; }
-          n1[r] = t1; ln1[r] = lt1; n2[r] = t2; ln2[r] = lt2;
+          n1 = t1; ln1 = lt1; n2 = t2; ln2 = lt2;
         }
         const int i = i0 + mfma_row(lane, r);
         const int j = j0 + (lane & 31);
         if (i < a.n_events && j < a.n_events && j >= i) {
           const long long idx = (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
-          const bool kp = p1[r] > a.threshold && p1[r] > 0.f;
-          const bool kn = n1[r] < -a.threshold && n1[r] < 0.f;
-          a.vpos[idx] = kp ? p1[r] : 0.f;
-          a.lpos[idx] = kp ? lp1[r] : 0;
-          a.vneg[idx] = kn ? n1[r] : 0.f;
-          a.lneg[idx] = kn ? ln1[r] : 0;
-          const bool tp = kp && (p1[r] - p2[r] < a.tie_tol || p1[r] < a.tie_tol);
-          const bool tn = kn && (n2[r] - n1[r] < a.tie_tol || -n1[r] < a.tie_tol);
+          const bool kp = p1 > a.threshold && p1 > 0.f;
+          const bool kn = n1 < -a.threshold && n1 < 0.f;
+          a.vpos[idx] = kp ? p1 : 0.f;
+          a.lpos[idx] = kp ? lp1 : 0;
+          a.vneg[idx] = kn ? n1 : 0.f;
+          a.lneg[idx] = kn ? ln1 : 0;
+          const bool tp = kp && (p1 - p2 < a.tie_tol || p1 < a.tie_tol);
+          const bool tn = kn && (n2 - n1 < a.tie_tol || -n1 < a.tie_tol);
           if (tp || tn) {
             const unsigned slot = atomicAdd(a.tie_count, 1u);
             if (slot < (unsigned)a.tie_cap) {
               a.ties[2 * slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0), 0);
-              a.ties[2 * slot + 1] = make_int4(lp1[r], lp2[r], ln1[r], ln2[r]);
+              a.ties[2 * slot + 1] = make_int4(lp1, lp2, ln1, ln2);
             }
           }
         }
