@@ -124,7 +124,7 @@ struct sxc_plan {
   bool full = false;
   long long spec_stride = 0;  // float2 per (event, channel) row
   long long n_pairs = 0;
-  int n_work = 0, chunk = 128, tie_cap = 0;  // chunk: columns per work item
+  int n_work = 0, chunk = 64, tie_cap = 0;  // chunk: columns per work item
   float tie_tol = 1e-5f;
   // device workspace
   float2* spec = nullptr;
@@ -455,7 +455,8 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   pa.ties = p->ties;
   pa.tie_count = p->tie_count;
   pa.tie_cap = p->tie_cap;
-  const int grid = std::min(p->n_work, 4096);
+  // a multiple of 8 (one share per XCD) whenever there are 8 items or more
+  const int grid = p->n_work >= 4096 ? 4096 : (p->n_work >= 8 ? p->n_work / 8 * 8 : p->n_work);
   HIP_CHECK(launch_pairs(p->R, p->full, pa, grid, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
 

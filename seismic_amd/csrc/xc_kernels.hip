@@ -310,7 +310,14 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
 
   stage_twiddles<R>(twl, a.tw, TwLayout<R>::kInv, threadIdx.x, blockDim.x);
 
-  for (int item = blockIdx.x; item < a.n_work; item += gridDim.x) {
+  // XCD-aware walk: workgroups are dispatched round-robin over the 8 XCDs,
+  // so XCD x = blockIdx % 8 takes its own contiguous eighth of the
+  // chunk-major work list and keeps one column chunk hot in its L2.
+  const int n_xcd = (gridDim.x % 8 == 0) ? 8 : 1;  // the host sizes the grid to a multiple of 8
+  const int xcd = blockIdx.x % n_xcd, local = blockIdx.x / n_xcd, n_local = gridDim.x / n_xcd;
+  const int seg = (a.n_work + n_xcd - 1) / n_xcd;
+  const int seg_end = min(a.n_work, (xcd + 1) * seg);
+  for (int item = xcd * seg + local; item < seg_end; item += n_local) {
     const int2 wk = a.work[item];
     const int i = wk.x;
     const int jb = wk.y;

@@ -207,10 +207,11 @@ __device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsign
   }
 }
 
-// Running per-pair state of one wave, in LDS: [value/lag][pos1, pos2, neg1, neg2]
+// Running per-pair state of one wave, in LDS, lane-contiguous so every
+// access is bank-conflict free: [wave][pair r][pos1, pos2, neg1, neg2][lane]
 struct LagState {
-  float v[4][64][16][4];
-  int l[4][64][16][4];
+  float v[4][16][4][64];
+  int l[4][16][4][64];
 };
 
 template <bool BF16X3>
@@ -233,9 +234,9 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     __syncthreads();  // the previous tile's merge has read the state
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      st.v[wv][lane][r][0] = 0.f; st.v[wv][lane][r][1] = -3.0e38f;
-      st.v[wv][lane][r][2] = 0.f; st.v[wv][lane][r][3] = 3.0e38f;
-      st.l[wv][lane][r][0] = 0; st.l[wv][lane][r][1] = 0; st.l[wv][lane][r][2] = 0; st.l[wv][lane][r][3] = 0;
+      st.v[wv][r][0][lane] = 0.f; st.v[wv][r][1][lane] = -3.0e38f;
+      st.v[wv][r][2][lane] = 0.f; st.v[wv][r][3][lane] = 3.0e38f;
+      st.l[wv][r][0][lane] = 0; st.l[wv][r][1][lane] = 0; st.l[wv][r][2][lane] = 0; st.l[wv][r][3][lane] = 0;
     }
     for (int b = wv; b < a.n_blocks; b += 4) {
       const int lb = a.lag_first + kLagBlock * b;
@@ -244,10 +245,10 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
       // top-2 per pair over the block's lags, in increasing lag order
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        float p1 = st.v[wv][lane][r][0], p2 = st.v[wv][lane][r][1];
-        float n1 = st.v[wv][lane][r][2], n2 = st.v[wv][lane][r][3];
-        int lp1 = st.l[wv][lane][r][0], lp2 = st.l[wv][lane][r][1];
-        int ln1 = st.l[wv][lane][r][2], ln2 = st.l[wv][lane][r][3];
+        float p1 = st.v[wv][r][0][lane], p2 = st.v[wv][r][1][lane];
+        float n1 = st.v[wv][r][2][lane], n2 = st.v[wv][r][3][lane];
+        int lp1 = st.l[wv][r][0][lane], lp2 = st.l[wv][r][1][lane];
+        int ln1 = st.l[wv][r][2][lane], ln2 = st.l[wv][r][3][lane];
 #pragma unroll
         for (int d = 0; d < kLagBlock; ++d) {
           const int lag = lb + d;
@@ -266,10 +267,10 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
           n1 = gn ? v : n1;
           ln1 = gn ? lag : ln1;
         }
-        st.v[wv][lane][r][0] = p1; st.v[wv][lane][r][1] = p2;
-        st.v[wv][lane][r][2] = n1; st.v[wv][lane][r][3] = n2;
-        st.l[wv][lane][r][0] = lp1; st.l[wv][lane][r][1] = lp2;
-        st.l[wv][lane][r][2] = ln1; st.l[wv][lane][r][3] = ln2;
+        st.v[wv][r][0][lane] = p1; st.v[wv][r][1][lane] = p2;
+        st.v[wv][r][2][lane] = n1; st.v[wv][r][3][lane] = n2;
+        st.l[wv][r][0][lane] = lp1; st.l[wv][r][1][lane] = lp2;
+        st.l[wv][r][2][lane] = ln1; st.l[wv][r][3][lane] = ln2;
       }
     }
     // merge the 4 waves' states (lag blocks interleave, so ties go to the smaller lag)
@@ -277,13 +278,13 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     if (wv == 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        float p1 = st.v[0][lane][r][0], p2 = st.v[0][lane][r][1];
-        float n1 = st.v[0][lane][r][2], n2 = st.v[0][lane][r][3];
-        int lp1 = st.l[0][lane][r][0], lp2 = st.l[0][lane][r][1];
-        int ln1 = st.l[0][lane][r][2], ln2 = st.l[0][lane][r][3];
+        float p1 = st.v[0][r][0][lane], p2 = st.v[0][r][1][lane];
+        float n1 = st.v[0][r][2][lane], n2 = st.v[0][r][3][lane];
+        int lp1 = st.l[0][r][0][lane], lp2 = st.l[0][r][1][lane];
+        int ln1 = st.l[0][r][2][lane], ln2 = st.l[0][r][3][lane];
         for (int w = 1; w < 4; ++w) {
-          const float q1 = st.v[w][lane][r][0], q2 = st.v[w][lane][r][1];
-          const int lq1 = st.l[w][lane][r][0], lq2 = st.l[w][lane][r][1];
+          const float q1 = st.v[w][r][0][lane], q2 = st.v[w][r][1][lane];
+          const int lq1 = st.l[w][r][0][lane], lq2 = st.l[w][r][1][lane];
           // positive side; the starting 0 at lag 0 loses every tie
           const bool takeq = q1 > p1 || (q1 == p1 && q1 > 0.f && lq1 < lp1);
           float s1, s2;
@@ -294,8 +295,8 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
           const int lo2 = p2 >= q2 ? lp2 : lq2;
           if (o2 > s2) { s2 = o2; ls2 = lo2; }
           p1 = s1; lp1 = ls1; p2 = s2; lp2 = ls2;
-          const float m1 = st.v[w][lane][r][2], m2 = st.v[w][lane][r][3];
-          const int lm1 = st.l[w][lane][r][2], lm2 = st.l[w][lane][r][3];
+          const float m1 = st.v[w][r][2][lane], m2 = st.v[w][r][3][lane];
+          const int lm1 = st.l[w][r][2][lane], lm2 = st.l[w][r][3][lane];
           const bool takem = m1 < n1 || (m1 == n1 && m1 < 0.f && lm1 < ln1);
           float t1, t2;
           int lt1, lt2;
