@@ -52,7 +52,9 @@ def reference_f64(padded, padded_rev, L, S, threshold=0.0):
     out = {k: [] for k in ("pos", "lagp", "neg", "lagn", "gapp", "gapn", "rawpos", "rawneg")}
     for i in range(n):
         ct = np.fft.irfft(F[i][None, :] * Rv[i:], n=P, axis=1) * P
-        w = ct[:, L - S:L + S] / (P * nr[i] * nr[i:, None])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            # a zero-energy event gives NaN here; the scan then yields (0, 0)
+            w = ct[:, L - S:L + S] / (P * nr[i] * nr[i:, None])
         pos, lagp, neg, lagn = scan_window(w, S)
         gp, gn = second_gap(w, S)
         keep_p = pos > threshold
