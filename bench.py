@@ -110,6 +110,52 @@ def lag_forms(ev, tr, stream, steps, warmup):
     return out
 
 
+def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed_rate: float):
+    """Serving form: a 3-channel 2048-sample catalog resident in HBM; requests
+    of 1..256 events with lengths U[512, 4096] at Poisson 200 req/s, top-10
+    matches each; then 64-event requests at a fixed Poisson rate for the
+    per-request latency."""
+    from seismic_amd import serving, synth
+    log(f"bench: serving catalog {n_catalog} x 3 x 2048")
+    cat, _ = synth.make_events(n_catalog, 3, 2048, seed=777)
+    t0 = time.perf_counter()
+    srv = serving.Server(cat, max_query_len=4096, top_k=10, max_batch_events=256, n_slots=4)
+    t_load = time.perf_counter() - t0
+    del cat
+    # warm-up: one request per period bucket
+    warm = serving.make_requests(4, 3, (512, 4096), (1, 64), seed=5, pool_events=64)
+    for i in [srv.submit(e, ln) for e, ln in warm]:
+        srv.wait(i)
+    out = {"catalog_events": n_catalog, "channels": 3, "catalog_samples": 2048, "top_k": 10,
+           "slots": srv.stats()["slots"], "slot_bytes": srv.stats()["slot_bytes"],
+           "catalog_load_s": round(t_load, 3)}
+
+    def leg(reqs, rate, seed):
+        srv.reset_stats()
+        r = serving.run_poisson(srv, reqs, rate, seed)
+        st = srv.stats()
+        lat = r["latency_ms"]
+        ev = int(r["sizes"].sum())
+        return {"requests": len(reqs), "events": ev, "offered_req_per_s": rate,
+                "completed_req_per_s": round(len(reqs) / r["wall_s"], 2),
+                "events_per_s": round(ev / r["wall_s"], 1),
+                "pairs_per_s": round(ev * n_catalog / r["wall_s"], 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "max_ms": round(float(lat.max()), 2),
+                "mean_requests_per_batch": round(st["mean_requests_per_batch"], 2),
+                "wall_s": round(r["wall_s"], 3)}
+
+    reqs = serving.make_requests(stream_requests, 3, (512, 4096), (1, 256), seed=11)
+    log(f"bench: serving stream, {stream_requests} requests at 200 req/s")
+    out["stream_1_256_events"] = leg(reqs, 200.0, 12)
+    reqs = serving.make_requests(fixed_requests, 3, (512, 4096), (64, 64), seed=13)
+    log(f"bench: serving 64-event requests, {fixed_requests} at {fixed_rate} req/s")
+    out["requests_64_events"] = leg(reqs, fixed_rate, 14)
+    srv.close()
+    return out
+
+
 def cpu_baseline(ev: np.ndarray, P: int, S: int, sample_events: int, threads: int):
     from oracle import oracle
     n = min(sample_events, ev.shape[0])
@@ -137,6 +183,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lag-forms", action="store_true", help="skip the MFMA lag-window forms")
     ap.add_argument("--lag-steps", type=int, default=2)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-catalog", type=int, default=32768)
+    ap.add_argument("--serve-stream-requests", type=int, default=64)
+    ap.add_argument("--serve-64-requests", type=int, default=60)
+    ap.add_argument("--serve-64-rate", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (scales PMC byte counters)")
     args = ap.parse_args()
@@ -197,6 +248,12 @@ def main():
         log(f"bench: bf16x3 {lag['bf16x3']['ms_per_step']} ms, fp16 {lag['fp16']['ms_per_step']} ms, "
             f"fft {lag['fft_same_window']['ms_per_step']} ms")
 
+    serve = None
+    if not args.no_serving:
+        serve = serving_leg(args.serve_catalog, args.serve_stream_requests, args.serve_64_requests,
+                            args.serve_64_rate)
+        log(f"bench: serving {json.dumps(serve)}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -217,7 +274,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mfma_bf16x3": lag,
-        "serving": None,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
     plan.close()

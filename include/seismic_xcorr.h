@@ -115,6 +115,62 @@ int sxc_device_synchronize(void);
  * used to scale profiler HBM counters. */
 int sxc_copy_kernel(void* dst, const void* src, unsigned long long bytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Serving: a catalog resident in HBM, matched against requests of new events
+ * with continuous batching.  No counterpart in the reference (a one-shot
+ * all-pairs call); it reuses the FP32-FFT kernels in query x catalog form.
+ * Each request (1..max_batch_events events, uneven lengths up to
+ * max_query_len) gets, per event, the top_k catalog events by largest
+ * coefficient, with lags (lag > 0: the query event is delayed relative to
+ * the catalog event; full linear lag range, FP32 argmax).
+ * n_slots worker slots each reserve device memory for one batch and own a
+ * HIP stream; a free slot takes every queued request that fits (FIFO) and
+ * runs them as one batch, so batches overlap on the device.  mem_budget
+ * (bytes, 0 = no limit) caps the slots: memory-based admission.  Results are
+ * independent of how requests are batched (per-pair computation).
+ * ------------------------------------------------------------------------ */
+typedef struct sxc_server sxc_server;
+
+typedef struct {
+  int n_catalog;          /* catalog events */
+  int n_channels;         /* channels per event, averaged per lag */
+  int catalog_len;        /* samples per catalog trace */
+  int max_query_len;      /* longest query trace accepted */
+  int top_k;              /* matches returned per query event, 1..32 */
+  int max_batch_events;   /* events per batch (and per request), e.g. 256 */
+  int n_slots;            /* batches in flight */
+  unsigned long long mem_budget;  /* device bytes for the slots, 0 = no limit */
+  int demean;             /* FP64 mean removal of every trace */
+} sxc_server_desc;
+
+typedef struct {
+  float value;            /* channel-averaged max coefficient */
+  int lag;
+  int index;              /* catalog event, -1 when fewer than top_k exist */
+} sxc_match;
+
+typedef struct {
+  unsigned long long requests, events, batches, queued;
+  int slots;
+  double p50_ms, p99_ms, max_ms;     /* submit -> results on the host */
+  double mean_requests_per_batch;
+  double events_per_s;               /* completed events / (last completion - first submit) */
+  unsigned long long slot_bytes;     /* device memory reserved per slot */
+} sxc_serve_stats;
+
+/* catalog: host [n_catalog][n_channels][catalog_len] float32 (copied; the
+ * catalog spectra are computed once and stay resident). */
+int sxc_server_create(const sxc_server_desc* desc, const float* catalog, sxc_server** out);
+/* events: host [n_events][n_channels][stride] float32, lengths[n_events]
+ * samples each (copied; returns at once). */
+int sxc_server_submit(sxc_server* server, const float* events, long long stride, const int* lengths,
+                      int n_events, long long* request_id);
+/* Blocks until the request is done; out: [n_events][top_k] matches. */
+int sxc_server_wait(sxc_server* server, long long request_id, sxc_match* out, double* latency_ms);
+int sxc_server_stats(sxc_server* server, sxc_serve_stats* out);
+int sxc_server_reset_stats(sxc_server* server);
+int sxc_server_destroy(sxc_server* server);
+
 /* Host-memory convenience wrapper: copies traces in, runs, copies the packed
  * outputs back.  traces: [n_events][n_channels][trace_stride] float32. */
 int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,

@@ -44,6 +44,40 @@ class SxcTiming(ctypes.Structure):
     ]
 
 
+class SxcServerDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_catalog", ctypes.c_int),
+        ("n_channels", ctypes.c_int),
+        ("catalog_len", ctypes.c_int),
+        ("max_query_len", ctypes.c_int),
+        ("top_k", ctypes.c_int),
+        ("max_batch_events", ctypes.c_int),
+        ("n_slots", ctypes.c_int),
+        ("mem_budget", ctypes.c_ulonglong),
+        ("demean", ctypes.c_int),
+    ]
+
+
+class SxcMatch(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_float), ("lag", ctypes.c_int), ("index", ctypes.c_int)]
+
+
+class SxcServerStats(ctypes.Structure):
+    _fields_ = [
+        ("requests", ctypes.c_ulonglong),
+        ("events", ctypes.c_ulonglong),
+        ("batches", ctypes.c_ulonglong),
+        ("queued", ctypes.c_ulonglong),
+        ("slots", ctypes.c_int),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("mean_requests_per_batch", ctypes.c_double),
+        ("events_per_s", ctypes.c_double),
+        ("slot_bytes", ctypes.c_ulonglong),
+    ]
+
+
 # every symbol include/*.h declares, per library
 EXPORTS = {
     "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
@@ -52,7 +86,9 @@ EXPORTS = {
                             "sxc_device_alloc", "sxc_device_free", "sxc_memcpy_h2d",
                             "sxc_memcpy_d2h", "sxc_memset_d8", "sxc_stream_create",
                             "sxc_stream_destroy", "sxc_stream_synchronize",
-                            "sxc_device_synchronize", "sxc_copy_kernel"],
+                            "sxc_device_synchronize", "sxc_copy_kernel", "sxc_server_create",
+                            "sxc_server_submit", "sxc_server_wait", "sxc_server_stats",
+                            "sxc_server_reset_stats", "sxc_server_destroy"],
     "correlation_cuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
     "correlation_multicuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
 }
@@ -104,6 +140,20 @@ def load_engine() -> ctypes.CDLL:
         f = getattr(lib, name)
         f.restype = ctypes.c_int
         f.argtypes = args
+    lib.sxc_server_create.restype = ctypes.c_int
+    lib.sxc_server_create.argtypes = [ctypes.POINTER(SxcServerDesc), c_float_p, ctypes.POINTER(vp)]
+    lib.sxc_server_submit.restype = ctypes.c_int
+    lib.sxc_server_submit.argtypes = [vp, c_float_p, ctypes.c_longlong, c_int_p, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_longlong)]
+    lib.sxc_server_wait.restype = ctypes.c_int
+    lib.sxc_server_wait.argtypes = [vp, ctypes.c_longlong, ctypes.POINTER(SxcMatch),
+                                    ctypes.POINTER(ctypes.c_double)]
+    lib.sxc_server_stats.restype = ctypes.c_int
+    lib.sxc_server_stats.argtypes = [vp, ctypes.POINTER(SxcServerStats)]
+    lib.sxc_server_reset_stats.restype = ctypes.c_int
+    lib.sxc_server_reset_stats.argtypes = [vp]
+    lib.sxc_server_destroy.restype = ctypes.c_int
+    lib.sxc_server_destroy.argtypes = [vp]
     lib.sxc_allpairs_host.restype = ctypes.c_int
     lib.sxc_allpairs_host.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, ctypes.c_longlong,
                                       ctypes.c_int, c_float_p, c_int_p, c_float_p, c_int_p]

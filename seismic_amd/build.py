@@ -70,7 +70,7 @@ def build(force: bool = False, verbose: bool = True, jobs: int | None = None) ->
     if force or _stale(out, [src] + hdrs):
         steps.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", out])
     hobjs = []
-    for name in ("xc_engine.cpp", "xc_dispatch.cpp"):
+    for name in ("xc_engine.cpp", "xc_dispatch.cpp", "xc_serve.cpp"):
         src = os.path.join(CSRC, name)
         out = os.path.join(OBJ, name.replace(".cpp", ".o"))
         hobjs.append(out)

@@ -48,7 +48,7 @@ const char* last_error() { return g_last_error.c_str(); }
 
 // exp(sign * 2 pi i k / n) in FP64 with the angle reduced to an octant, then
 // rounded once to FP32 (same construction as the device's compile-time table).
-static void twiddle(long k, long n, int sign, float* re, float* im) {
+void twiddle(long k, long n, int sign, float* re, float* im) {
   k %= n;
   if (k < 0) k += n;
   long k8 = 8 * k, oct = k8 / n, rem = k8 - oct * n;
@@ -159,10 +159,12 @@ static void plan_free(sxc_plan* p) {
   delete p;
 }
 
+namespace sxc {
+
 // Twiddle tables in the device layout of TwLayout<R> (xc_fft.hpp):
 //   [k2*64 + t] = W_M^(t*k2), k2 < R;  [64R + k*8 + t1] = W_64^(t1*k);
 //   inverse only: [64R + 64 + m*64 + t] = exp(+2 pi i (t + 64 m) / P).
-static std::vector<float2> wave_tables(int R, int sign) {
+std::vector<float2> wave_tables(int R, int sign) {
   const int M = 64 * R, P = 2 * M;
   const int kW = 64 * R, kPre = 64 * R + 64;
   std::vector<float2> t(sign > 0 ? kPre + 64 * R : kPre);
@@ -170,22 +172,37 @@ static std::vector<float2> wave_tables(int R, int sign) {
     for (int l = 0; l < 64; ++l) twiddle((long)l * k, M, sign, &t[k * 64 + l].x, &t[k * 64 + l].y);
   for (int k = 0; k < 8; ++k)
     for (int l = 0; l < 8; ++l) twiddle((long)l * k, 64, sign, &t[kW + k * 8 + l].x, &t[kW + k * 8 + l].y);
-  if (sign > 0)
+  if (sign > 0) {
     for (int m = 0; m < R; ++m)
       for (int l = 0; l < 64; ++l) twiddle(l + 64L * m, P, +1, &t[kPre + m * 64 + l].x, &t[kPre + m * 64 + l].y);
+    // compact inverse tables (TwLayoutCT) follow the full ones
+    const size_t b = t.size();
+    t.resize(b + 640 + 8 * R);
+    for (int k = 0; k < 8; ++k)
+      for (int l = 0; l < 64; ++l) twiddle((long)l * k, M, +1, &t[b + k * 64 + l].x, &t[b + k * 64 + l].y);
+    for (int a = 0; a < R / 8; ++a)
+      for (int l = 0; l < 64; ++l)
+        twiddle(8L * l * a, M, +1, &t[b + 512 + a * 64 + l].x, &t[b + 512 + a * 64 + l].y);
+    for (int k = 0; k < 8; ++k)
+      for (int l = 0; l < 8; ++l)
+        twiddle((long)l * k, 64, +1, &t[b + 512 + 8 * R + k * 8 + l].x, &t[b + 512 + 8 * R + k * 8 + l].y);
+    for (int l = 0; l < 64; ++l) twiddle(l, P, +1, &t[b + 576 + 8 * R + l].x, &t[b + 576 + 8 * R + l].y);
+  }
   return t;
 }
 
 // Lag held by (lane, slot) after the inverse wave FFT: slot = 2*reg + {re, im},
 // reg = 8s + k holds frequency index q = (lane/8 + 8s) + R*(lane%8 + 8k),
 // i.e. samples 2q and 2q+1 of the period-P correlation.
-static int slot_lag_host(int R, int lane, int slot) {
+int slot_lag_host(int R, int lane, int slot) {
   const int M = 64 * R;
   const int reg = slot >> 1;
   const int q = ((lane >> 3) + 8 * (reg >> 3)) + R * ((lane & 7) + 8 * (reg & 7));
   const int n = 2 * q + (slot & 1);
   return n <= M ? n : n - 2 * M;
 }
+
+}  // namespace sxc
 
 static int upload_tables(sxc_plan* p) {
   const int R = p->R, M = p->M, P = p->P;

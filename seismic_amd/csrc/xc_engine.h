@@ -1,8 +1,18 @@
 // Internal helpers shared by the engine and the drop-in ABI shims.
 #pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <vector>
+
 #include "../../include/seismic_xcorr.h"
 
 namespace sxc {
+// exp(sign * 2 pi i k / n), FP64 with octant reduction, rounded once to FP32
+void twiddle(long k, long n, int sign, float* re, float* im);
+// twiddle tables of the one-wave FFT of R values per lane (device layout)
+std::vector<float2> wave_tables(int R, int sign);
+// lag held by (lane, slot) after the inverse wave FFT
+int slot_lag_host(int R, int lane, int slot);
 void set_error(const char* fmt, ...);
 void clear_error();
 const char* last_error();

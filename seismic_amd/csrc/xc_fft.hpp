@@ -184,6 +184,22 @@ struct TwLayout {
   static constexpr int kInv = kPre + 64 * R; // entries the pair kernels need
 };
 
+// Compact inverse tables, for pair kernels whose staged row spectra leave no
+// room for the full ones (stored right after them, at offset kInv):
+//   step 2:  A[b][t] = W_M^(t*b), b < 8, and B[a][t] = W_M^(8*t*a), a < R/8,
+//            so W_M^(t*k2) = A[k2 % 8] * B[k2 / 8]            (512 + 8R entries)
+//   w64:     as in the full tables                          (64 entries)
+//   pre:     u[t] = exp(+2 pi i t / P); bin t + 64m uses u[t] times the
+//            compile-time exp(+2 pi i m / 2R)                (64 entries)
+template <int R>
+struct TwLayoutCT {
+  static constexpr int kS2A = 0;
+  static constexpr int kS2B = 512;
+  static constexpr int kW64 = 512 + 8 * R;
+  static constexpr int kPreL = kW64 + 64;
+  static constexpr int kInv = kPreL + 64;
+};
+
 template <int R>
 __device__ __forceinline__ void stage_twiddles(float2* lds_tw, const float2* __restrict__ tab,
                                                int n, int tid, int nthreads) {
@@ -192,19 +208,38 @@ __device__ __forceinline__ void stage_twiddles(float2* lds_tw, const float2* __r
 
 // The four-step wave FFT (see file comment).  `lds` is this wave's private
 // kWaveLds-float2 buffer, `tw` the staged twiddle tables.
-template <int R, int SIGN>
+// CT selects the compact tables (tw then points at TwLayoutCT's layout).
+template <int R, int SIGN, bool CT = false>
 __device__ __forceinline__ void wave_fft(cf (&d)[R], const float2* tw, float2* lds, int lane) {
   static_assert(R % 8 == 0, "R must be a multiple of 8");
   using TL = TwLayout<R>;
+  using TC = TwLayoutCT<R>;
   reg_fft<R, SIGN>(d);
+  if constexpr (CT) {
+    sfor<0, R / 8>([&](auto ac) {
+      constexpr int a = decltype(ac)::value;
+      const cf wb = a ? f2c(tw[TC::kS2B + a * 64 + lane]) : cmk(1.f, 0.f);
+      sfor<0, 8>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        if constexpr (b == 0 && a > 0) {
+          d[8 * a] = cmul(d[8 * a], wb);
+        } else if constexpr (b > 0) {
+          const cf wa = f2c(tw[TC::kS2A + b * 64 + lane]);
+          d[8 * a + b] = cmul(d[8 * a + b], a ? cmul(wa, wb) : wa);
+        }
+      });
+    });
+  } else {
 #pragma unroll
-  for (int k = 1; k < R; ++k) d[k] = cmul(d[k], f2c(tw[TL::kS2 + k * 64 + lane]));
+    for (int k = 1; k < R; ++k) d[k] = cmul(d[k], f2c(tw[TL::kS2 + k * 64 + lane]));
+  }
 
   const int t1 = lane & 7;
   const int g = lane >> 3;
   cf w64[8];
+  constexpr int kW64 = CT ? TC::kW64 : TL::kW64;
 #pragma unroll
-  for (int k = 1; k < 8; ++k) w64[k] = f2c(tw[TL::kW64 + k * 8 + t1]);
+  for (int k = 1; k < 8; ++k) w64[k] = f2c(tw[kW64 + k * 8 + t1]);
   // transpose 1 + radix-8 over t2, chunk by chunk
   sfor<0, R / 8>([&](auto sc) {
     constexpr int s = decltype(sc)::value;

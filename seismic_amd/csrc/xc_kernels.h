@@ -20,12 +20,17 @@ struct ForwardArgs {
   long long spec_stride;
   const float2* tw;       // forward twiddle tables (TwLayout)
   const float2* post_tw;  // exp(-2 pi i k / P), k = 0..M
+  const int* row_len;     // optional per-event sample counts (rows of one event share it);
+                          // when set it replaces in_len and L for that row
 };
 
 struct PairArgs {
-  const float2* spec;     // [n_events][n_channels][spec_stride]
+  const float2* spec;     // row spectra [n_rows][n_channels][spec_stride]
   long long spec_stride;
-  int n_events;
+  const float2* col_spec; // column spectra (null: the row spectra, all-pairs mode)
+  int n_events;           // all-pairs: events; rectangular: columns
+  int dense_cols;         // 0: packed upper triangle; > 0: out[i * dense_cols + j], all (i, j)
+  int pos_only;           // 1: write only vpos / lpos
   int n_channels;
   const int2* work;       // (row i, first column) per work item
   int n_work;
@@ -33,7 +38,7 @@ struct PairArgs {
   int lag_lo, lag_hi;     // scanned lags, inclusive
   float threshold;
   float tie_tol;
-  const float2* tw;       // inverse twiddle tables (TwLayout, incl. the C2R packing table)
+  const float2* tw;       // inverse twiddle tables (TwLayout, then TwLayoutCT)
   const unsigned long long* winmask;  // [2R] lane masks of in-window slots
   float* vpos;
   int* lpos;
@@ -78,10 +83,16 @@ hipError_t launch_pairs(int R, bool full, const PairArgs& a, int grid, hipStream
 hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st);
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
 hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream_t st);
+// top-k of each row of a dense [n_rows][n_cols] score matrix, by value
+// (descending; equal values by smaller column); rows with fewer than k
+// columns pad with (-inf, 0, -1).
+hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int k, float* out_v,
+                       int* out_l, int* out_idx, hipStream_t st);
 
-// twiddle tables (128R + 64) + staged row spectra + one transpose buffer per wave
-inline size_t pair_lds_bytes(int R, int n_channels, int waves) {
-  return (size_t)(128 * R + 64 + n_channels * (64 * R + 1) + waves * 576) * 8;
+// twiddle tables (full: 128R + 64, compact: 640 + 8R) + staged row spectra +
+// one transpose buffer per wave
+inline size_t pair_lds_bytes(int R, int n_channels, int waves, bool compact = false) {
+  return (size_t)((compact ? 640 + 8 * R : 128 * R + 64) + n_channels * (64 * R + 1) + waves * 576) * 8;
 }
 
 }  // namespace sxc

@@ -69,6 +69,16 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   wave_lds_fence();
 
   const float* x = a.in + (long long)row * a.in_stride;
+  if (a.row_len) {  // uneven lengths: this row's event sets both counts
+    const int n = a.row_len[row / a.n_channels];
+    a.in_len = n < (int)a.in_stride ? n : (int)a.in_stride;
+    a.L = a.in_len;
+  }
+  if (a.in_len < 1) {  // empty row: zero spectrum
+    for (int k = lane; k <= M; k += 64) a.spec[(long long)row * a.spec_stride + k] = make_float2(0.f, 0.f);
+    if (lane == 0) a.norms[row] = 0.0;
+    return;
+  }
   // mean and energy of the first L samples, FP64, straight from memory
   double s1 = 0.0;
   for (int n = lane; n < a.L; n += 64) s1 += (double)(n < a.in_len ? x[n] : 0.f);
@@ -139,7 +149,7 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
 // half-length sequence Z[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/P}
 // (Y[k] - conj Y[M-k]) and runs the inverse wave FFT.  On return lane l,
 // register r holds c[2q] (x) and c[2q+1] (y), q = wave_out_index(l, r).
-template <int R, bool XI_GLOBAL = false>
+template <int R, bool XI_GLOBAL = false, bool CT = false>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
                                              long long xj_stride, int nch, const float2* tw,
@@ -182,6 +192,13 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
 
   const int src = (64 - lane) & 63;
   const float2* pre = tw + TwLayout<R>::kPre + lane;
+  const cf u = CT ? f2c(tw[TwLayoutCT<R>::kPreL + lane]) : cmk(0.f, 0.f);
+  // exp(+2 pi i (lane + 64 m) / P)
+  auto pre_tw = [&](auto mc) -> cf {
+    constexpr int mm = decltype(mc)::value;
+    if constexpr (CT) return twmul<mm, 2 * R, 1>(u);
+    else return f2c(pre[mm * 64]);
+  };
   cf saved = cmk(yN, 0.f);  // lane 0: original y[R - m]
   // Z = E + i T D with E = Y + conj(Ym), D = Y - conj(Ym), T = e^{2 pi i k/P}
   auto zform = [](cf a, cf am, cf t) {
@@ -208,10 +225,10 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
       bm.y = l0 ? nb.y : bm.y;
     }
     saved = b;
-    y[m] = zform(a, am, f2c(pre[m * 64]));
-    y[mh] = zform(b, bm, f2c(pre[mh * 64]));
+    y[m] = zform(a, am, pre_tw(std::integral_constant<int, m>{}));
+    y[mh] = zform(b, bm, pre_tw(std::integral_constant<int, mh>{}));
   });
-  wave_fft<R, 1>(y, tw, wl, lane);
+  wave_fft<R, 1, CT>(y, tw, wl, lane);
 }
 
 // lag of (lane, slot) where slot = 2*reg + {0: x, 1: y}
@@ -297,18 +314,24 @@ __device__ __forceinline__ long long packed_index(int i, int j, int n) {
 // ----------------------------------------------------------------------------
 // pair kernel: one workgroup = 4 waves per work item (row i, column chunk)
 // ----------------------------------------------------------------------------
-template <int R, bool FULL, int kWaves>
+// XI_LDS: the row spectra of a work item are staged in LDS (shared by the
+// workgroup's waves); otherwise (when they do not fit next to the tables,
+// e.g. P = 8192 with 3 channels) every wave reads them from global memory.
+// CT: compact twiddle tables (they leave room for the row spectra of larger
+// P x channel counts at the price of one extra complex multiply per twiddle).
+template <int R, bool FULL, int kWaves, bool XI_LDS, bool CT>
 __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
   constexpr int M = 64 * R;
+  constexpr int kTab = CT ? TwLayoutCT<R>::kInv : TwLayout<R>::kInv;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int xs = M + 1;  // LDS row stride of the staged row spectra
+  const int xs = XI_LDS ? M + 1 : (int)a.spec_stride;  // row stride of the row spectra
   float2* twl = smem;
-  float2* xi_lds = smem + TwLayout<R>::kInv;
-  float2* wl = xi_lds + a.n_channels * xs + wv * kWaveLds;
+  float2* xi_lds = smem + kTab;
+  float2* wl = xi_lds + (XI_LDS ? a.n_channels * xs : 0) + wv * kWaveLds;
 
-  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kInv, threadIdx.x, blockDim.x);
+  stage_twiddles<R>(twl, a.tw + (CT ? TwLayout<R>::kInv : 0), kTab, threadIdx.x, blockDim.x);
 
   // XCD-aware walk: workgroups are dispatched round-robin over the 8 XCDs,
   // so XCD x = blockIdx % 8 takes its own contiguous eighth of the
@@ -322,29 +345,35 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
     const int i = wk.x;
     const int jb = wk.y;
     const int je = min((jb / a.chunk + 1) * a.chunk, a.n_events);  // end of jb's column chunk
-    __syncthreads();
+    const bool dense = a.dense_cols > 0;
     const float2* src = a.spec + (long long)i * a.n_channels * a.spec_stride;
-    for (int c = 0; c < a.n_channels; ++c)
-      for (int k = threadIdx.x; k <= M; k += blockDim.x)
-        xi_lds[c * xs + k] = src[c * a.spec_stride + k];
-    __syncthreads();
+    if constexpr (XI_LDS) {
+      __syncthreads();
+      for (int c = 0; c < a.n_channels; ++c)
+        for (int k = threadIdx.x; k <= M; k += blockDim.x)
+          xi_lds[c * xs + k] = src[c * a.spec_stride + k];
+      __syncthreads();
+    }
 
-    for (int j = max(jb, i) + wv; j < je; j += kWaves) {
+    const float2* cols = a.col_spec ? a.col_spec : a.spec;
+    for (int j = (dense ? jb : max(jb, i)) + wv; j < je; j += kWaves) {
       cf y[R];
-      const float2* xj = a.spec + (long long)j * a.n_channels * a.spec_stride;
-      pair_inverse<R>(y, xi_lds, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
+      const float2* xj = cols + (long long)j * a.n_channels * a.spec_stride;
+      pair_inverse<R, !XI_LDS, CT>(y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
       bool tp, tn;
       scan_wave<R>(y, lane, a.tie_tol, a.threshold, pos, neg, tp, tn);
       if (lane == 0) {
-        const long long idx = packed_index(i, j, a.n_events);
+        const long long idx = dense ? (long long)i * a.dense_cols + j : packed_index(i, j, a.n_events);
         const bool kp = pos.v > a.threshold, kn = neg.v < -a.threshold;
         a.vpos[idx] = kp ? pos.v : 0.f;
         a.lpos[idx] = kp ? pos.lag : 0;
-        a.vneg[idx] = kn ? neg.v : 0.f;
-        a.lneg[idx] = kn ? neg.lag : 0;
-        if (tp || tn) {
+        if (!a.pos_only) {
+          a.vneg[idx] = kn ? neg.v : 0.f;
+          a.lneg[idx] = kn ? neg.lag : 0;
+        }
+        if ((tp || tn) && a.tie_cap > 0) {
           unsigned slot = atomicAdd(a.tie_count, 1u);
           if (slot < (unsigned)a.tie_cap) a.ties[slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0), 0);
         }
@@ -464,23 +493,34 @@ hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int R, bool FULL, int W>
+template <int R, bool FULL, int W, bool XI_LDS, bool CT>
 static hipError_t launch_pairs_rfw(const PairArgs& a, int grid, hipStream_t st) {
-  size_t lds = pair_lds_bytes(R, a.n_channels, W);
+  size_t lds = pair_lds_bytes(R, XI_LDS ? a.n_channels : 0, W, CT);
   static_assert(TwLayout<R>::kInv == 128 * R + 64, "twiddle layout");
-  hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL, W>,
+  static_assert(TwLayoutCT<R>::kInv == 640 + 8 * R, "compact twiddle layout");
+  hipError_t e = hipFuncSetAttribute((const void*)k_pairs<R, FULL, W, XI_LDS, CT>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_pairs<R, FULL, W>), dim3(grid), dim3(W * 64), lds, st, a);
+  hipLaunchKernelGGL((k_pairs<R, FULL, W, XI_LDS, CT>), dim3(grid), dim3(W * 64), lds, st, a);
   return hipGetLastError();
 }
 
 template <int R, bool FULL>
 static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
-  // 12 waves (3 per SIMD) when their LDS fits the 160 KiB of a CU, else 8
-  if (pair_lds_bytes(R, a.n_channels, 12) <= 160 * 1024)
-    return launch_pairs_rfw<R, FULL, 12>(a, grid, st);
-  return launch_pairs_rfw<R, FULL, 8>(a, grid, st);
+  // 12 waves (3 per SIMD) when the staged row spectra fit the 160 KiB of a
+  // CU, else 8; then 8 waves with the compact tables (P >= 4096 only); rows
+  // from global memory when none of these fit.  P = 8192 always runs 8
+  // waves: its 128 spectrum registers per lane need the 256-VGPR budget.
+  constexpr size_t kLds = 160 * 1024;
+  const int C = a.n_channels;
+  if constexpr (R < 64) {
+    if (pair_lds_bytes(R, C, 12, false) <= kLds) return launch_pairs_rfw<R, FULL, 12, true, false>(a, grid, st);
+  }
+  if (pair_lds_bytes(R, C, 8, false) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, false>(a, grid, st);
+  if constexpr (R >= 32) {
+    if (pair_lds_bytes(R, C, 8, true) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, true>(a, grid, st);
+  }
+  return launch_pairs_rfw<R, FULL, 8, false, false>(a, grid, st);
 }
 
 template <int R>
@@ -514,6 +554,85 @@ hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
+  return hipGetLastError();
+}
+
+// One workgroup (256 threads) per row: each thread keeps a sorted top-k of a
+// strided share of the columns in registers, then wave 0 merges the 256
+// candidate lists with k rounds of a wave-wide argmax.
+constexpr int kTopkMax = 32;
+__device__ __forceinline__ bool better(float v, int c, float bv, int bc) {
+  return v > bv || (v == bv && c < bc);
+}
+__global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const int* __restrict__ l,
+                                              int n_cols, int k, float* out_v, int* out_l, int* out_idx) {
+  __shared__ float cv[256 * kTopkMax];
+  __shared__ int cc[256 * kTopkMax];
+  const int row = blockIdx.x;
+  const float* rv = v + (long long)row * n_cols;
+  float tv[kTopkMax];
+  int tc[kTopkMax];
+#pragma unroll
+  for (int q = 0; q < kTopkMax; ++q) { tv[q] = -INFINITY; tc[q] = 0x7fffffff; }
+  for (int c = threadIdx.x; c < n_cols; c += 256) {
+    float x = rv[c];
+    int xc = c;
+#pragma unroll
+    for (int q = 0; q < kTopkMax; ++q) {  // insertion, keeps tv sorted descending
+      if (q < k && better(x, xc, tv[q], tc[q])) {
+        const float sv = tv[q];
+        const int sc = tc[q];
+        tv[q] = x; tc[q] = xc;
+        x = sv; xc = sc;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kTopkMax; ++q) {
+    cv[threadIdx.x * kTopkMax + q] = q < k ? tv[q] : -INFINITY;
+    cc[threadIdx.x * kTopkMax + q] = q < k ? tc[q] : 0x7fffffff;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  int head[4] = {0, 0, 0, 0};  // next unused candidate of the 4 lists this lane owns
+  for (int r = 0; r < k; ++r) {
+    float bv = -INFINITY;
+    int bc = 0x7fffffff, bl = -1;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = lane + 64 * u;
+      if (head[u] < k) {
+        const float x = cv[t * kTopkMax + head[u]];
+        const int xc = cc[t * kTopkMax + head[u]];
+        if (better(x, xc, bv, bc)) { bv = x; bc = xc; bl = u; }
+      }
+    }
+    // wave argmax over (bv, bc)
+    float wv = bv;
+    int wc = bc;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ov = __shfl_xor(wv, o, 64);
+      const int oc = __shfl_xor(wc, o, 64);
+      if (better(ov, oc, wv, wc)) { wv = ov; wc = oc; }
+    }
+    if (bl >= 0 && bc == wc && bv == wv) head[bl]++;  // the owner advances its list
+    if (lane == 0) {
+      const long long o = (long long)row * k + r;
+      const bool ok = wc != 0x7fffffff;
+      out_v[o] = ok ? wv : -INFINITY;
+      out_idx[o] = ok ? wc : -1;
+      out_l[o] = ok ? l[(long long)row * n_cols + wc] : 0;
+    }
+  }
+}
+
+hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int k, float* out_v,
+                       int* out_l, int* out_idx, hipStream_t st) {
+  if (n_rows <= 0) return hipSuccess;
+  if (k < 1 || k > kTopkMax) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_topk, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
   return hipGetLastError();
 }
 

@@ -174,3 +174,26 @@ def test_extended_demean_and_short_rows(oracle):
     assert np.abs(vn - ref["neg"]).max() <= COEF_TOL
     sure = ref["gap_pos"] >= GPU_EXACT_GAP
     assert np.array_equal(lp[sure], ref["lag_pos"][sure])
+
+
+@pytest.mark.parametrize("C,full", [(1, False), (2, False), (3, False), (3, True), (4, False)])
+def test_extended_p8192_channel_counts(oracle, C, full):
+    """P=8192 with 1..4 channels: 8-wave kernel with full twiddle tables
+    (C=1), with compact tables (C=2, 3) and with rows read from global
+    memory (C=4)."""
+    from seismic_amd import engine
+    n, L, P = 16, 1500, 8192
+    ev, _ = synth.make_events(n, C, L, seed=60 + C)
+    lo, hi = (-P // 2 + 1, P // 2) if full else (-1600, 1600)
+    print(f"case P=8192 C={C} window [{lo}, {hi}]", flush=True)
+    vp, lp, vn, ln = engine.allpairs_host(ev, P, lo, hi)
+    pi, pj = oracle.packed_pairs(n)
+    # lags beyond +-L hold exact zeros, which never win the strict scan
+    ref = oracle.pairs_f64(ev, pi, pj, -1600, 1600, P=P, circular=True)
+    dp, dn = np.abs(vp - ref["pos"]).max(), np.abs(vn - ref["neg"]).max()
+    print(f"P=8192 C={C}: max |dcoef| pos {dp:.3e} neg {dn:.3e}")
+    assert dp <= COEF_TOL and dn <= COEF_TOL
+    sure_p = ref["gap_pos"] >= GPU_EXACT_GAP
+    sure_n = ref["gap_neg"] >= GPU_EXACT_GAP
+    assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
+    assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])

@@ -1,0 +1,127 @@
+"""Serving loop: argument checks on the CPU, results against an FP64 numpy
+oracle, batching independence and admission on the GPU."""
+import numpy as np
+import pytest
+
+from seismic_amd import serving, synth
+
+
+def ncc_f64(q, x):
+    """Channel-averaged linear NCC of query q [C, Lq] against catalog event
+    x [C, Lc] for lags -(Lc-1) .. Lq-1 (lag > 0: query delayed), FP64."""
+    C, Lq = q.shape
+    Lc = x.shape[1]
+    n = 1
+    while n < Lq + Lc - 1:
+        n *= 2
+    acc = np.zeros(Lq + Lc - 1)
+    for c in range(C):
+        a = q[c].astype(np.float64)
+        b = x[c].astype(np.float64)
+        na, nb = np.linalg.norm(a), np.linalg.norm(b)
+        if na == 0 or nb == 0:
+            continue
+        # r[l] = sum_m a[m] b[m - l]; index l + (Lc - 1)
+        r = np.fft.irfft(np.fft.rfft(a, n) * np.conj(np.fft.rfft(b, n)), n)
+        full = np.concatenate([r[n - (Lc - 1):], r[:Lq]])
+        acc += full / (na * nb)
+    return acc / C
+
+
+def oracle_topk(queries, lens, catalog, k):
+    out = []
+    Lc = catalog.shape[2]
+    for qi in range(len(queries)):
+        q = queries[qi][:, :lens[qi]]
+        best = []
+        for ci in range(catalog.shape[0]):
+            r = ncc_f64(q, catalog[ci])
+            m = int(np.argmax(r))
+            v = r[m]
+            srt = np.sort(r)
+            gap = srt[-1] - srt[-2]
+            best.append((v if v > 0 else 0.0, (m - (Lc - 1)) if v > 0 else 0, ci, gap))
+        best.sort(key=lambda t: (-t[0], t[2]))
+        out.append(best[:k])
+    return out
+
+
+def test_invalid_server_desc_rejected():
+    cat = np.zeros((4, 1, 100), np.float32)
+    with pytest.raises(RuntimeError, match="invalid server description"):
+        serving.Server(cat, max_query_len=100, top_k=40)
+    with pytest.raises(RuntimeError, match="memory budget"):
+        serving.Server(cat, max_query_len=100, top_k=2, mem_budget=1000)
+
+
+@pytest.mark.gpu
+def test_serving_matches_f64():
+    cat, _ = synth.make_events(150, 3, 512, seed=21)
+    srv = serving.Server(cat, max_query_len=900, top_k=5, max_batch_events=32, n_slots=2)
+    rng = np.random.default_rng(22)
+    pool, _ = synth.make_events(12, 3, 900, seed=23)
+    lens = rng.integers(100, 901, size=12).astype(np.int32)
+    lens[0], lens[1] = 513, 514     # both sides of the P=1024 / P=2048 boundary
+    q = pool.copy()
+    for e in range(12):
+        q[e, :, lens[e]:] = 0
+    # a query that is an exact (shifted) copy of a catalog event
+    q[2] = 0
+    q[2, :, 30:30 + 512] = cat[77]
+    lens[2] = 600
+    rid = srv.submit(q, lens)
+    v, l, idx, lat = srv.wait(rid)
+    ref = oracle_topk(q, lens, cat, 5)
+    worst = 0.0
+    for e in range(12):
+        rv = np.array([t[0] for t in ref[e]])
+        worst = max(worst, np.abs(v[e] - rv).max())
+        for r in range(5):
+            sep = (r == 0 or rv[r - 1] - rv[r] > 1e-5) and (r == 4 or rv[r] - rv[r + 1] > 1e-5)
+            if sep:
+                assert idx[e, r] == ref[e][r][2], (e, r)
+                if ref[e][r][3] > 1e-5:
+                    assert l[e, r] == ref[e][r][1], (e, r)
+    print(f"serving: max |dcoef| {worst:.3e}, latency {lat:.1f} ms")
+    assert worst <= 1e-5
+    assert idx[2, 0] == 77 and l[2, 0] == 30 and abs(v[2, 0] - 1) < 1e-5
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_result_same_alone_and_in_mix():
+    cat, _ = synth.make_events(300, 3, 1024, seed=31)
+    srv = serving.Server(cat, max_query_len=2000, top_k=10, max_batch_events=64, n_slots=3)
+    reqs = serving.make_requests(24, 3, (200, 2000), (1, 40), seed=32, pool_events=64)
+    ev, lens = reqs[7]
+    alone = srv.wait(srv.submit(ev, lens))
+    ids = [srv.submit(e, ln) for e, ln in reqs]          # all in flight together
+    mixed = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    for a, b in zip(alone[:3], mixed[7][:3]):
+        assert np.array_equal(a, b)
+    assert st["batches"] < 25          # some requests were batched together
+    print("requests per batch", st["mean_requests_per_batch"], "slots", st["slots"])
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_memory_budget_limits_slots():
+    cat, _ = synth.make_events(64, 1, 256, seed=41)
+    one = serving.Server(cat, max_query_len=256, top_k=3, max_batch_events=8, n_slots=1)
+    per_slot = one.stats()["slot_bytes"]
+    one.close()
+    assert per_slot > 0
+    srv = serving.Server(cat, max_query_len=256, top_k=3, max_batch_events=8, n_slots=8,
+                         mem_budget=int(2.5 * per_slot))
+    assert srv.stats()["slots"] == 2
+    reqs = serving.make_requests(10, 1, (50, 256), (1, 8), seed=42, pool_events=16)
+    ids = [srv.submit(e, ln) for e, ln in reqs]
+    for i in ids:
+        srv.wait(i)
+    assert srv.stats()["requests"] == 10
+    with pytest.raises(RuntimeError, match="events"):
+        srv.submit(np.zeros((9, 1, 256), np.float32), np.full(9, 100, np.int32))
+    with pytest.raises(RuntimeError, match="length"):
+        srv.submit(np.zeros((1, 1, 300), np.float32), np.array([300], np.int32))
+    srv.close()
