@@ -171,6 +171,44 @@ int sxc_server_stats(sxc_server* server, sxc_serve_stats* out);
 int sxc_server_reset_stats(sxc_server* server);
 int sxc_server_destroy(sxc_server* server);
 
+/* ------------------------------------------------------------------------
+ * Tiled N x N runs of the lag-window forms, for N past the packed int32
+ * triangle of the reference (Correlation_in_CUDA/correlation_cuda.cu:144:
+ * N(N+1)/2 > INT32_MAX from N = 65536) and outputs larger than device
+ * memory.  The upper-triangle blocks (row block I <= column block J) of
+ * block x block events are computed in row-major block order and streamed
+ * to the host as float16 coefficients and int16 lags while the next block
+ * computes.  A diagonal block is delivered whole: below its diagonal it
+ * holds the transposed entries (coefficient equal, lag negated), i.e. pair
+ * (j, i) over the mirrored window [-lag_hi, -lag_lo].  The blocks below the
+ * diagonal of the N x N matrix are the transposes of the delivered ones with
+ * lags negated.  Entries carry the same values as a packed-triangle execute
+ * of the same plan, rounded to float16 (round to nearest even). */
+typedef struct sxc_tiled sxc_tiled;
+
+typedef struct {
+  int index;       /* block number in delivery order, -1 once every block was delivered */
+  int n_blocks;    /* upper-triangle blocks in the run */
+  int row0, col0;  /* first event of the block's rows / columns */
+  int rows, cols;  /* block extent (smaller at the matrix edge) */
+} sxc_block_info;
+
+/* desc: a lag-window form (SXC_FORM_LAG_*), lags within int16; block: a
+ * multiple of 32 in [32, 16384]; want_neg: also deliver the negative side. */
+int sxc_tiled_create(const sxc_plan_desc* desc, int block, int want_neg, sxc_tiled** out);
+/* d_traces: device [n_events][n_channels][trace_stride] float32 (samples at
+ * index >= in_len are zero); read before the call returns.  Starts the
+ * pipeline (a run in flight is drained first). */
+int sxc_tiled_start(sxc_tiled* t, const float* d_traces, long long trace_stride, int in_len);
+/* Waits for the next block and copies it into host buffers of row stride ld
+ * elements (any pointer may be NULL to skip that output); info->index == -1
+ * when the run is complete. */
+int sxc_tiled_next(sxc_tiled* t, sxc_block_info* info, unsigned short* coef_pos, short* lag_pos,
+                   unsigned short* coef_neg, short* lag_neg, long long ld);
+/* pairs delivered so far (upper triangle incl. diagonal), near ties, overflow */
+int sxc_tiled_stats(sxc_tiled* t, sxc_stats* out);
+int sxc_tiled_destroy(sxc_tiled* t);
+
 /* Host-memory convenience wrapper: copies traces in, runs, copies the packed
  * outputs back.  traces: [n_events][n_channels][trace_stride] float32. */
 int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,

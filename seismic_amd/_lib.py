@@ -78,6 +78,11 @@ class SxcServerStats(ctypes.Structure):
     ]
 
 
+class SxcBlockInfo(ctypes.Structure):
+    _fields_ = [("index", ctypes.c_int), ("n_blocks", ctypes.c_int), ("row0", ctypes.c_int),
+                ("col0", ctypes.c_int), ("rows", ctypes.c_int), ("cols", ctypes.c_int)]
+
+
 # every symbol include/*.h declares, per library
 EXPORTS = {
     "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
@@ -88,7 +93,9 @@ EXPORTS = {
                             "sxc_stream_destroy", "sxc_stream_synchronize",
                             "sxc_device_synchronize", "sxc_copy_kernel", "sxc_server_create",
                             "sxc_server_submit", "sxc_server_wait", "sxc_server_stats",
-                            "sxc_server_reset_stats", "sxc_server_destroy"],
+                            "sxc_server_reset_stats", "sxc_server_destroy", "sxc_tiled_create",
+                            "sxc_tiled_start", "sxc_tiled_next", "sxc_tiled_stats",
+                            "sxc_tiled_destroy"],
     "correlation_cuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
     "correlation_multicuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
 }
@@ -154,6 +161,17 @@ def load_engine() -> ctypes.CDLL:
     lib.sxc_server_reset_stats.argtypes = [vp]
     lib.sxc_server_destroy.restype = ctypes.c_int
     lib.sxc_server_destroy.argtypes = [vp]
+    lib.sxc_tiled_create.restype = ctypes.c_int
+    lib.sxc_tiled_create.argtypes = [ctypes.POINTER(SxcPlanDesc), ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(vp)]
+    lib.sxc_tiled_start.restype = ctypes.c_int
+    lib.sxc_tiled_start.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int]
+    lib.sxc_tiled_next.restype = ctypes.c_int
+    lib.sxc_tiled_next.argtypes = [vp, ctypes.POINTER(SxcBlockInfo), vp, vp, vp, vp, ctypes.c_longlong]
+    lib.sxc_tiled_stats.restype = ctypes.c_int
+    lib.sxc_tiled_stats.argtypes = [vp, ctypes.POINTER(SxcStats)]
+    lib.sxc_tiled_destroy.restype = ctypes.c_int
+    lib.sxc_tiled_destroy.argtypes = [vp]
     lib.sxc_allpairs_host.restype = ctypes.c_int
     lib.sxc_allpairs_host.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, ctypes.c_longlong,
                                       ctypes.c_int, c_float_p, c_int_p, c_float_p, c_int_p]

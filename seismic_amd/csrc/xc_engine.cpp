@@ -20,6 +20,7 @@
 #include "xc_engine.h"
 #include "xc_kernels.h"
 #include "xc_lag.h"
+#include "xc_plan.h"
 
 namespace sxc {
 
@@ -37,14 +38,6 @@ void set_error(const char* fmt, ...) {
 void clear_error() { g_last_error.clear(); }
 const char* last_error() { return g_last_error.c_str(); }
 
-#define HIP_CHECK(expr)                                                              \
-  do {                                                                               \
-    hipError_t _e = (expr);                                                          \
-    if (_e != hipSuccess) {                                                          \
-      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
-      return SXC_ERR_HIP;                                                            \
-    }                                                                                \
-  } while (0)
 
 // exp(sign * 2 pi i k / n) in FP64 with the angle reduced to an octant, then
 // rounded once to FP32 (same construction as the device's compile-time table).
@@ -70,18 +63,6 @@ void twiddle(long k, long n, int sign, float* re, float* im) {
   *im = (float)(sign * s);
 }
 
-template <typename T>
-static int dev_alloc(T** p, size_t count, const char* what) {
-  *p = nullptr;
-  if (count == 0) count = 1;
-  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
-  if (e != hipSuccess) {
-    set_error("device allocation of %s (%zu bytes) failed: %s", what, count * sizeof(T), hipGetErrorString(e));
-    *p = nullptr;
-    return SXC_ERR_MEMORY;
-  }
-  return SXC_OK;
-}
 
 static bool is_lag_form(int form) { return form == SXC_FORM_LAG_BF16X3 || form == SXC_FORM_LAG_FP16; }
 
@@ -118,38 +99,8 @@ int validate_desc(const sxc_plan_desc& d) {
 
 using namespace sxc;
 
-struct sxc_plan {
-  sxc_plan_desc d;
-  int R = 0, M = 0, P = 0;
-  bool full = false;
-  long long spec_stride = 0;  // float2 per (event, channel) row
-  long long n_pairs = 0;
-  int n_work = 0, chunk = 64, tie_cap = 0;  // chunk: columns per work item
-  float tie_tol = 1e-5f;
-  // device workspace
-  float2* spec = nullptr;
-  double* norms = nullptr;
-  float* traces = nullptr;     // demeaned copy for the FP64 resolver (demean only)
-  int2* work = nullptr;
-  int4* ties = nullptr;
-  unsigned* tie_count = nullptr;
-  float2 *fwd2 = nullptr, *inv2 = nullptr, *post = nullptr;
-  unsigned long long* winmask = nullptr;
-  hipStream_t last_stream = nullptr;
-  int device = 0;
-  // lag-window form
-  bool lag = false;
-  int l16 = 0, p0 = 0, ls = 0, lag_first = 0, n_blocks = 0, n_tiles = 0;
-  unsigned short *hi = nullptr, *lo = nullptr;
-  float* lag_traces = nullptr;
-  int2* tiles = nullptr;
-  // stage timing: 4 events per execute (start, after forward, pairs, resolve)
-  bool timing = false;
-  std::vector<hipEvent_t> events;
-  int timed = 0;
-};
 
-static void plan_free(sxc_plan* p) {
+void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
@@ -241,7 +192,7 @@ static std::vector<int2> build_work(int n, int chunk) {
 
 // Lag-window form: zero-padded bf16 (hi, lo) or fp16 planes of every trace,
 // 32 x 32 pair tiles of the upper triangle, blocks of 8 lags.
-static int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
+int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   sxc_plan* p = new sxc_plan();
   p->d = *desc;
   p->lag = true;
@@ -286,13 +237,9 @@ static int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   return SXC_OK;
 }
 
-static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
-                            float* vp, int* lp, float* vn, int* ln, hipStream_t st) {
+int lag_plan_prep(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len, hipStream_t st) {
   const sxc_plan_desc& d = p->d;
   const int rows = d.n_events * d.n_channels;
-  hipEvent_t* ev = nullptr;
-  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
-  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
   LagPrepArgs pa{};
   pa.in = d_traces;
   pa.in_stride = trace_stride;
@@ -309,6 +256,17 @@ static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_
   pa.norms = p->norms;
   pa.traces_out = p->lag_traces;
   HIP_CHECK(launch_lag_prep(pa, st));
+  return SXC_OK;
+}
+
+static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                            float* vp, int* lp, float* vn, int* ln, hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
+  hipEvent_t* ev = nullptr;
+  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
+  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
+  int rc = lag_plan_prep(p, d_traces, trace_stride, in_len, st);
+  if (rc) return rc;
   HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
   LagPairArgs ka{};

@@ -35,6 +35,9 @@ struct LagPairArgs {
   int n_blocks;             // blocks of kLagBlock lags
   const int2* tiles;        // (row tile, column tile), column >= row
   int n_tiles;
+  int tile_row0, tile_col0; // added to every tile entry (block runs)
+  long long out_ld;         // 0: packed triangle output; else dense block with this row stride
+  int out_row0, out_col0;   // event index of the block's first row / column
   float threshold, tie_tol;
   float* vpos;
   int* lpos;
@@ -59,10 +62,25 @@ struct LagResolveArgs {
   int* lpos;
   float* vneg;
   int* lneg;
+  long long out_ld;         // as in LagPairArgs
+  int out_row0, out_col0;
+};
+
+// FP32 value / int32 lag block -> float16 coefficient / int16 lag block
+struct PackBlockArgs {
+  const float* v;
+  const int* l;
+  long long in_ld;
+  int rows, cols;
+  unsigned short* coef;     // float16 bits, [rows][cols]
+  short* lag;
+  int mirror;               // diagonal block: entries below the diagonal are the
+                            // transposed ones above it, lag negated
 };
 
 hipError_t launch_lag_prep(const LagPrepArgs& a, hipStream_t st);
 hipError_t launch_lag_pairs(bool bf16x3, const LagPairArgs& a, int grid, hipStream_t st);
 hipError_t launch_lag_resolve(const LagResolveArgs& a, int grid, hipStream_t st);
+hipError_t launch_pack_block(const PackBlockArgs& a, hipStream_t st);
 
 }  // namespace sxc

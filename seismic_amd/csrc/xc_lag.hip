@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
   __shared__ LagState st;
   for (int item = blockIdx.x; item < a.n_tiles; item += gridDim.x) {
     const int2 t = a.tiles[item];
-    const int i0 = 32 * t.x, j0 = 32 * t.y;
+    const int i0 = 32 * (t.x + a.tile_row0), j0 = 32 * (t.y + a.tile_col0);
+    if (i0 >= a.n_events || j0 >= a.n_events) continue;  // ragged last block (workgroup-uniform)
     // this lane's A row and B column (clamped: rows past N compute values that are never stored)
     const int ia = min(i0 + (lane & 31), a.n_events - 1);
     const int jb = min(j0 + (lane & 31), a.n_events - 1);
@@ -310,7 +311,8 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
         const int i = i0 + mfma_row(lane, r);
         const int j = j0 + (lane & 31);
         if (i < a.n_events && j < a.n_events && j >= i) {
-          const long long idx = (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
+          const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
+                                         : (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
           const bool kp = p1 > a.threshold && p1 > 0.f;
           const bool kn = n1 < -a.threshold && n1 < 0.f;
           a.vpos[idx] = kp ? p1 : 0.f;
@@ -352,7 +354,8 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
   for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
     const int4 t0 = a.ties[2 * rec], t1 = a.ties[2 * rec + 1];
     const int i = t0.x, j = t0.y, sides = t0.z;
-    const long long idx = (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
+    const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
+                                   : (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
     for (int sd = 0; sd < 2; ++sd) {
       if (!(sides & (1 << sd))) continue;
       const int c0 = sd == 0 ? t1.x : t1.z, c1 = sd == 0 ? t1.y : t1.w;
@@ -395,6 +398,21 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
 }
 
 // ----------------------------------------------------------------------------
+// block output: FP32 / int32 -> float16 / int16 (round to nearest even)
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_pack_block(PackBlockArgs a) {
+  const long long n = (long long)a.rows * a.cols;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const int r = (int)(e / a.cols), c = (int)(e - (long long)r * a.cols);
+    const bool below = a.mirror && c < r;
+    const long long src = below ? (long long)c * a.in_ld + r : (long long)r * a.in_ld + c;
+    const int lag = a.l[src];
+    a.coef[e] = __builtin_bit_cast(unsigned short, (_Float16)a.v[src]);
+    a.lag[e] = (short)(below ? -lag : lag);
+  }
+}
+
+// ----------------------------------------------------------------------------
 // launchers
 // ----------------------------------------------------------------------------
 hipError_t launch_lag_prep(const LagPrepArgs& a, hipStream_t st) {
@@ -409,6 +427,14 @@ hipError_t launch_lag_pairs(bool bf16x3, const LagPairArgs& a, int grid, hipStre
     hipLaunchKernelGGL(k_lag_pairs<true>, dim3(grid), dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(k_lag_pairs<false>, dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_block(const PackBlockArgs& a, hipStream_t st) {
+  const long long n = (long long)a.rows * a.cols;
+  if (n <= 0) return hipSuccess;
+  const long long g = (n + 255) / 256;
+  hipLaunchKernelGGL(k_pack_block, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

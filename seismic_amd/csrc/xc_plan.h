@@ -1,0 +1,70 @@
+// Plan state shared by the engine's translation units (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <vector>
+
+#include "../../include/seismic_xcorr.h"
+#include "xc_engine.h"
+
+#define HIP_CHECK(expr)                                                              \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+      return SXC_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+namespace sxc {
+template <typename T>
+inline int dev_alloc(T** p, size_t count, const char* what) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    set_error("device allocation of %s (%zu bytes) failed: %s", what, count * sizeof(T), hipGetErrorString(e));
+    *p = nullptr;
+    return SXC_ERR_MEMORY;
+  }
+  return SXC_OK;
+}
+}  // namespace sxc
+
+struct sxc_plan {
+  sxc_plan_desc d;
+  int R = 0, M = 0, P = 0;
+  bool full = false;
+  long long spec_stride = 0;  // float2 per (event, channel) row
+  long long n_pairs = 0;
+  int n_work = 0, chunk = 64, tie_cap = 0;  // chunk: columns per work item
+  float tie_tol = 1e-5f;
+  // device workspace
+  float2* spec = nullptr;
+  double* norms = nullptr;
+  float* traces = nullptr;     // demeaned copy for the FP64 resolver (demean only)
+  int2* work = nullptr;
+  int4* ties = nullptr;
+  unsigned* tie_count = nullptr;
+  float2 *fwd2 = nullptr, *inv2 = nullptr, *post = nullptr;
+  unsigned long long* winmask = nullptr;
+  hipStream_t last_stream = nullptr;
+  int device = 0;
+  // lag-window form
+  bool lag = false;
+  int l16 = 0, p0 = 0, ls = 0, lag_first = 0, n_blocks = 0, n_tiles = 0;
+  unsigned short *hi = nullptr, *lo = nullptr;
+  float* lag_traces = nullptr;
+  int2* tiles = nullptr;
+  // stage timing: 4 events per execute (start, after forward, pairs, resolve)
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  int timed = 0;
+};
+
+void plan_free(sxc_plan* p);
+// lag-window forms: planes, norms, FP32 traces, triangle tile list, tie records
+int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out);
+// lag-window forms: FP64 energy, scaled bf16 hi/lo (or fp16) planes and the
+// FP32 copy of every trace
+int lag_plan_prep(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len, hipStream_t st);

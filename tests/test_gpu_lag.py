@@ -68,3 +68,35 @@ def test_lag_exact_shift_duplicates():
         # packed (0,1)=1, (0,2)=2, (1,2)=4
         assert lp[1] == -41 and lp[2] == 77 and lp[4] == 118
         assert abs(vp[1] - 1) < tol and abs(vp[2] - 1) < tol and abs(vp[0] - 1) < tol
+
+
+def test_cfg3_forms_vs_f64_sampled(oracle):
+    """Precision configuration at full size: N=8192, 1 channel, L=4096,
+    window +-512, all 33.5M pairs in bf16x3, fp16 and the FP32-FFT form
+    (P=8192, circular lags equal the linear ones since P >= L + 512);
+    the FP64 oracle on 10000 seeded pairs.  Prints the coefficient error and
+    the lag agreement rate of every form."""
+    n, L, w = 8192, 4096, 512
+    ev, _ = synth.make_events(n, 1, L, seed=8192)
+    rng = np.random.Generator(np.random.PCG64(3))
+    k = 10000
+    pi = rng.integers(0, n, size=k)
+    pj = rng.integers(0, n, size=k)
+    pi, pj = np.minimum(pi, pj).astype(np.int32), np.maximum(pi, pj).astype(np.int32)
+    idx = pi.astype(np.int64) * n - pi.astype(np.int64) * (pi - 1) // 2 + (pj - pi)
+    ref = oracle.pairs_f64(ev, pi, pj, -w, w, circular=False)
+    keep = ref["pos"] > 1e-3
+    for name, form, P, coef_tol, gap in (("bf16x3", engine.FORM_LAG_BF16X3, 0, 1e-5, 4e-5),
+                                         ("fp16", engine.FORM_LAG_FP16, 0, 2e-3, 1e-3),
+                                         ("fft_f32", engine.FORM_FFT_F32, 8192, 1e-5, 2e-6)):
+        print(f"case cfg3 {name}", flush=True)
+        vp, lp, vn, ln = engine.allpairs_host(ev, P, -w, w, form=form)
+        dp = np.abs(vp[idx] - ref["pos"]).max()
+        dn = np.abs(vn[idx] - ref["neg"]).max()
+        agree = (lp[idx][keep] == ref["lag_pos"][keep]).mean()
+        print(f"cfg3 {name}: max |dcoef| pos {dp:.3e} neg {dn:.3e}; lag agreement {agree:.6f} "
+              f"on {int(keep.sum())} sampled pairs")
+        assert dp <= coef_tol and dn <= coef_tol
+        sure = keep & (ref["gap_pos"] >= gap)
+        assert np.array_equal(lp[idx][sure], ref["lag_pos"][sure])
+        del vp, lp, vn, ln
