@@ -156,6 +156,45 @@ def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed
     return out
 
 
+def tiled_leg(n_events: int, block: int):
+    """Tiled N x N form at the largest-run trace shape: 1 channel, L=4096,
+    window +-1024, bf16x3, log-uniform SNR [0.3, 30], 10% shifted duplicates;
+    float16 coefficient + int16 lag blocks streamed to host memory."""
+    from seismic_amd import synth, tiled
+    from seismic_amd.device import DeviceArray
+    L, w = 4096, 1024
+    log(f"bench: tiled N={n_events} L={L} +-{w} block {block}")
+    ev, _ = synth.make_events(n_events, 1, L, seed=65536, snr=(0.3, 30.0), log_snr=True, dup_frac=0.1)
+    tr = DeviceArray.from_numpy(ev)
+    del ev
+    run = tiled.TiledRun(n_events, 1, L, -w, w, block=block)
+    e = run.edge
+    bufs = [np.empty((e, e), np.float16), np.empty((e, e), np.int16)]
+    t0 = time.perf_counter()
+    run.start(tr)
+    nb = 0
+    t_first = None
+    while run.next_block(bufs) is not None:
+        nb += 1
+        if t_first is None:
+            t_first = time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    st = run.stats()
+    run.close()
+    tr.free()
+    pairs = n_events * (n_events + 1) // 2
+    rate = pairs / wall
+    issued = 3 * 2 * (2 * w + 1) * L  # bf16x3 MFMA flops per pair
+    n5 = 65536
+    return {"n_events": n_events, "samples": L, "lags": [-w, w], "form": "bf16x3", "block": block,
+            "blocks": nb, "pairs": pairs, "wall_s": round(wall, 3),
+            "first_block_s": round(t_first or 0.0, 3), "pairs_per_s": round(rate, 1),
+            "mfma_tflops_issued": round(rate * issued / 1e12, 1),
+            "output_bytes_to_host": pairs * 4, "near_ties": st["near_ties"],
+            "tie_overflow": st["tie_overflow"],
+            "projected_s_n65536": round(n5 * (n5 + 1) / 2 / rate, 1)}
+
+
 def cpu_baseline(ev: np.ndarray, P: int, S: int, sample_events: int, threads: int):
     from oracle import oracle
     n = min(sample_events, ev.shape[0])
@@ -184,6 +223,10 @@ def main():
     ap.add_argument("--no-lag-forms", action="store_true", help="skip the MFMA lag-window forms")
     ap.add_argument("--lag-steps", type=int, default=2)
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-tiled", action="store_true", help="skip the tiled N x N leg")
+    ap.add_argument("--tiled-n", type=int, default=16384,
+                    help="events of the tiled leg (65536 = the full largest run)")
+    ap.add_argument("--tiled-block", type=int, default=8192)
     ap.add_argument("--serve-catalog", type=int, default=32768)
     ap.add_argument("--serve-stream-requests", type=int, default=64)
     ap.add_argument("--serve-64-requests", type=int, default=60)
@@ -254,6 +297,11 @@ def main():
                             args.serve_64_rate)
         log(f"bench: serving {json.dumps(serve)}")
 
+    tiles = None
+    if not args.no_tiled:
+        tiles = tiled_leg(args.tiled_n, args.tiled_block)
+        log(f"bench: tiled {json.dumps(tiles)}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
@@ -275,6 +323,7 @@ def main():
         "cpu_baseline": cpu,
         "mfma_bf16x3": lag,
         "serving": serve,
+        "tiled_nxn": tiles,
     }
     print(json.dumps(line), flush=True)
     plan.close()

@@ -98,6 +98,7 @@ EXPORTS = {
                             "sxc_tiled_destroy"],
     "correlation_cuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
     "correlation_multicuda.so": ["initialiseCUDA", "correlationCUDA", "seismic_last_error"],
+    "correlation_c.so": ["correlationCPP", "seismic_last_error"],
 }
 
 

@@ -55,7 +55,8 @@ def build(force: bool = False, verbose: bool = True, jobs: int | None = None) ->
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(OBJ, exist_ok=True)
     hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [
-        os.path.join(HERE, "..", "include", "seismic_xcorr.h")]
+        os.path.join(HERE, "..", "include", h) for h in
+        ("seismic_xcorr.h", "correlation_cuda.h", "correlation_multicuda.h", "correlation_c.h")]
     steps = []
     kobjs = []
     for r in KERNEL_RADICES:
@@ -85,7 +86,8 @@ def build(force: bool = False, verbose: bool = True, jobs: int | None = None) ->
              ["-o", core + ".tmp"])
         os.replace(core + ".tmp", core)
     for shim, src in (("correlation_cuda.so", "abi_cuda.cpp"),
-                      ("correlation_multicuda.so", "abi_multicuda.cpp")):
+                      ("correlation_multicuda.so", "abi_multicuda.cpp"),
+                      ("correlation_c.so", "abi_omp.cpp")):
         out = os.path.join(LIB, shim)
         s = os.path.join(CSRC, src)
         if force or _stale(out, [s, core] + hdrs):

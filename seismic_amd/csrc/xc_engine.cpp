@@ -622,6 +622,88 @@ void reference_call(float* events, int n_events, int event_length, int shift, in
   sxc_allpairs_host(&d, events, padded, padded, vp, lp, vn, ln);
 }
 
+// OMP ABI (Correlation_in_OMP/correlation_c.cpp:209-298): the same job, plus
+// the per-class histograms of every pair's pos and neg coefficient taken
+// before the threshold and added (+=) into the caller's arrays (:284-288).
+// The pairs run without a threshold; the histogram pass then applies it.
+void reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
+                        float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+                        int* max_hist, int* min_hist) {
+  clear_error();
+  if (!events || !vp || !lp || !vn || !ln || !max_hist || !min_hist) {
+    set_error("null pointer argument");
+    return;
+  }
+  if (n_classes < 1) {
+    set_error("n_classes_hist must be >= 1 (got %d)", n_classes);
+    return;
+  }
+  sxc_plan_desc d;
+  if (reference_desc(n_events, event_length, shift, padded, threshold, &d)) return;
+  const long long np = (long long)n_events * (n_events + 1) / 2;
+  if (shift == 0) {  // empty scan: every pair is (0, lag 0), class 0 on both sides
+    memset(vp, 0, np * sizeof(float));
+    memset(lp, 0, np * sizeof(int));
+    memset(vn, 0, np * sizeof(float));
+    memset(ln, 0, np * sizeof(int));
+    max_hist[0] += (int)np;
+    min_hist[0] += (int)np;
+    return;
+  }
+  d.threshold = -INFINITY;
+  sxc_plan* p = nullptr;
+  if (sxc_plan_create(&d, &p)) return;
+  float *d_in = nullptr, *d_vp = nullptr, *d_vn = nullptr;
+  int *d_lp = nullptr, *d_ln = nullptr, *d_hist = nullptr;
+  auto cleanup = [&]() {
+    void* bufs[] = {d_in, d_vp, d_vn, d_lp, d_ln, d_hist};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    sxc_plan_destroy(p);
+  };
+  const size_t n = (size_t)np;
+  if (dev_alloc(&d_in, (size_t)n_events * padded, "input traces") || dev_alloc(&d_vp, n, "output values") ||
+      dev_alloc(&d_vn, n, "output values") || dev_alloc(&d_lp, n, "output lags") ||
+      dev_alloc(&d_ln, n, "output lags") || dev_alloc(&d_hist, 2 * (size_t)n_classes, "histograms")) {
+    cleanup();
+    return;
+  }
+  std::vector<int> h(2 * (size_t)n_classes);
+  bool ok = hipMemcpy(d_in, events, (size_t)n_events * padded * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemset(d_hist, 0, 2 * (size_t)n_classes * sizeof(int)) == hipSuccess;
+  if (ok) ok = sxc_plan_execute(p, d_in, padded, padded, d_vp, d_lp, d_vn, d_ln, nullptr) == SXC_OK;
+  if (ok) {
+    HistArgs ha{};
+    ha.vpos = d_vp;
+    ha.lpos = d_lp;
+    ha.vneg = d_vn;
+    ha.lneg = d_ln;
+    ha.n = np;
+    ha.n_classes = n_classes;
+    ha.delta = (float)(1.0 / n_classes);
+    ha.threshold = threshold;
+    ha.hist_pos = d_hist;
+    ha.hist_neg = d_hist + n_classes;
+    ok = launch_hist_threshold(ha, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    if (!ok) set_error("histogram pass failed");
+  }
+  if (ok)
+    ok = hipMemcpy(vp, d_vp, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(lp, d_lp, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(vn, d_vn, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(ln, d_ln, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(h.data(), d_hist, h.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+  if (ok) {
+    for (int k = 0; k < n_classes; ++k) {
+      max_hist[k] += h[k];
+      min_hist[k] += h[n_classes + k];
+    }
+  } else if (!last_error()[0]) {
+    set_error("device execution or copy failed");
+  }
+  cleanup();
+}
+
 int warm_up() {
   clear_error();
   int n = 0;
@@ -721,6 +803,13 @@ int sxc_device_synchronize(void) {
 void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
                         float threshold, float* vp, int* lp, float* vn, int* ln) {
   sxc::reference_call(events, n_events, event_length, shift, padded, threshold, vp, lp, vn, ln);
+}
+
+void sxc_reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
+                            float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+                            int* max_hist, int* min_hist) {
+  sxc::reference_call_omp(events, n_events, event_length, shift, padded, threshold, n_classes, vp, lp, vn,
+                          ln, max_hist, min_hist);
 }
 int sxc_warm_up(void) { return sxc::warm_up(); }
 }

@@ -23,6 +23,9 @@ int reference_desc(int n_events, int event_length, int shift, int padded, float 
 // Full drop-in call on host buffers; errors are reported via last_error().
 void reference_call(float* events, int n_events, int event_length, int shift, int padded,
                     float threshold, float* vp, int* lp, float* vn, int* ln);
+void reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
+                        float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+                        int* max_hist, int* min_hist);
 int warm_up();
 }  // namespace sxc
 
@@ -30,5 +33,8 @@ extern "C" {
 // Exported for the drop-in shims (same library namespace).
 void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
                         float threshold, float* vp, int* lp, float* vn, int* ln);
+void sxc_reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
+                            float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+                            int* max_hist, int* min_hist);
 int sxc_warm_up(void);
 }

@@ -89,6 +89,22 @@ hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream
 hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int k, float* out_v,
                        int* out_l, int* out_idx, hipStream_t st);
 
+// Histograms and threshold over packed results computed without a threshold
+// (the OMP ABI's histograms count every pair before its threshold).
+struct HistArgs {
+  float* vpos;
+  int* lpos;
+  float* vneg;
+  int* lneg;
+  long long n;            // packed pairs
+  int n_classes;
+  float delta;            // float(1.0 / n_classes)
+  float threshold;
+  int* hist_pos;          // [n_classes], accumulated with atomics
+  int* hist_neg;
+};
+hipError_t launch_hist_threshold(const HistArgs& a, hipStream_t st);
+
 // twiddle tables (full: 128R + 64, compact: 640 + 8R) + staged row spectra +
 // one transpose buffer per wave
 inline size_t pair_lds_bytes(int R, int n_channels, int waves, bool compact = false) {

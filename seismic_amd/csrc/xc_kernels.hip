@@ -41,6 +41,58 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 }
 
 #if SXC_R == 8
+// Histogram + threshold pass (Correlation_in_OMP/correlation_c.cpp:114-136):
+// pos value v -> class v >= 1 ? n-1 : int(v / delta), neg value u -> class
+// u <= -1 ? n-1 : int(-u / delta) (FP32 division, truncation), counted before
+// the threshold; then pos kept if v > threshold, neg if u < -threshold, else
+// (0, 0).  The class index is clamped to n-1 (the reference can index one
+// past the end when v / delta rounds up to n).  Block-local histograms in LDS
+// when 2 n ints fit 64 KiB, else direct global atomics.
+__global__ __launch_bounds__(256) void k_hist_threshold(HistArgs a) {
+  extern __shared__ int hs[];
+  const int nc = a.n_classes;
+  const bool lds = 2 * nc <= 16384;
+  if (lds) {
+    for (int k = threadIdx.x; k < 2 * nc; k += 256) hs[k] = 0;
+    __syncthreads();
+  }
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < a.n; e += (long long)gridDim.x * 256) {
+    const float v = a.vpos[e], u = a.vneg[e];
+    const int bp = v >= 1.f ? nc - 1 : max(0, min((int)__fdiv_rn(v, a.delta), nc - 1));
+    const int bn = u <= -1.f ? nc - 1 : max(0, min((int)__fdiv_rn(-u, a.delta), nc - 1));
+    if (lds) {
+      atomicAdd(&hs[bp], 1);
+      atomicAdd(&hs[nc + bn], 1);
+    } else {
+      atomicAdd(&a.hist_pos[bp], 1);
+      atomicAdd(&a.hist_neg[bn], 1);
+    }
+    if (!(v > a.threshold)) {
+      a.vpos[e] = 0.f;
+      a.lpos[e] = 0;
+    }
+    if (!(u < -a.threshold)) {
+      a.vneg[e] = 0.f;
+      a.lneg[e] = 0;
+    }
+  }
+  if (lds) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < nc; k += 256) {
+      if (hs[k]) atomicAdd(&a.hist_pos[k], hs[k]);
+      if (hs[nc + k]) atomicAdd(&a.hist_neg[k], hs[nc + k]);
+    }
+  }
+}
+
+hipError_t launch_hist_threshold(const HistArgs& a, hipStream_t st) {
+  if (a.n <= 0 || a.n_classes < 1) return hipSuccess;
+  const long long g = (a.n + 255) / 256;
+  const size_t lds = 2 * a.n_classes <= 16384 ? (size_t)2 * a.n_classes * sizeof(int) : 0;
+  hipLaunchKernelGGL(k_hist_threshold, dim3((unsigned)(g < 2048 ? g : 2048)), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;

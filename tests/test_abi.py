@@ -22,7 +22,8 @@ def _declared(header):
 
 @pytest.mark.parametrize("lib,header", [("libseismic_xcorr.so", "seismic_xcorr.h"),
                                          ("correlation_cuda.so", "correlation_cuda.h"),
-                                         ("correlation_multicuda.so", "correlation_multicuda.h")])
+                                         ("correlation_multicuda.so", "correlation_multicuda.h"),
+                                         ("correlation_c.so", "correlation_c.h")])
 def test_exports(lib, header):
     path = _lib.library_path(lib)
     assert os.path.exists(path), "build the libraries first (python -m seismic_amd.build)"
@@ -89,3 +90,19 @@ def test_invalid_plan_reported():
 def test_multicuda_binding_loads():
     lib = multicuda_lib.initialise_library()
     assert hasattr(lib, "correlationCUDA") and hasattr(lib, "initialiseCUDA")
+
+
+def test_omp_binding_loads_and_rejects_bad_classes():
+    from seismic_amd import omp_lib
+    lib = omp_lib.initialize_library()
+    n, P = 3, 1024
+    ev = np.zeros((n, P), np.float32)
+    outs = [np.full(6, 7, np.float32), np.full(6, 7, np.int32), np.full(6, 7, np.float32),
+            np.full(6, 7, np.int32)]
+    hist = [np.zeros(4, np.int32), np.zeros(4, np.int32)]
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    lib.correlationCPP(fp(ev), fp(ev), n, 500, 100, P, 4, 1, 0.0, 0, fp(outs[0]), ip(outs[1]),
+                       fp(outs[2]), ip(outs[3]), ip(hist[0]), ip(hist[1]))
+    assert b"n_classes_hist" in lib.seismic_last_error()
+    assert all((o == 7).all() for o in outs) and not hist[0].any()
