@@ -201,21 +201,21 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
 // half-length sequence Z[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/P}
 // (Y[k] - conj Y[M-k]) and runs the inverse wave FFT.  On return lane l,
 // register r holds c[2q] (x) and c[2q+1] (y), q = wave_out_index(l, r).
-template <int R, bool XI_GLOBAL = false, bool CT = false>
+template <int R, bool XI_GLOBAL = false, bool CT = false, int kLoadGroup = 32>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
                                              long long xj_stride, int nch, const float2* tw,
                                              float2* wl, int lane) {
   constexpr int M = 64 * R;
-#pragma unroll
-  for (int m = 0; m < R; ++m) y[m] = cmk(0.f, 0.f);
   float yN = 0.f;
-  for (int c = 0; c < nch; ++c) {
+  // channel 0 initialises the accumulators, channels 1.. add to them
+  auto channel = [&](int c, auto first_c) {
+    constexpr bool kFirst = decltype(first_c)::value;
     const float2* ai = xi + c * xi_stride;
     const float2* bj = xj + c * xj_stride;
     // All of a group's global loads are issued before the compiler fence so
     // they are in flight together (one L2 round trip per group, not per row).
-    constexpr int kGroup = R < 32 ? R : 32;
+    constexpr int kGroup = R < kLoadGroup ? R : kLoadGroup;
 #pragma unroll
     for (int m0 = 0; m0 < R; m0 += kGroup) {
       float2 Bv[kGroup];
@@ -232,13 +232,20 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
         const int m = m0 + u;
         cf A = f2c(XI_GLOBAL ? Av[XI_GLOBAL ? u : 0] : ai[lane + 64 * m]);
         cf B = f2c(Bv[u]);
-        y[m].x = __builtin_fmaf(A.x, B.x, __builtin_fmaf(A.y, B.y, y[m].x));
-        y[m].y = __builtin_fmaf(A.y, B.x, __builtin_fmaf(-A.x, B.y, y[m].y));
+        if constexpr (kFirst) {
+          y[m].x = __builtin_fmaf(A.x, B.x, A.y * B.y);
+          y[m].y = __builtin_fmaf(A.y, B.x, -A.x * B.y);
+        } else {
+          y[m].x = __builtin_fmaf(A.x, B.x, __builtin_fmaf(A.y, B.y, y[m].x));
+          y[m].y = __builtin_fmaf(A.y, B.x, __builtin_fmaf(-A.x, B.y, y[m].y));
+        }
       }
     }
     float2 aN = ai[M], bN = bj[M];
-    yN = __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
-  }
+    yN = kFirst ? __builtin_fmaf(aN.x, bN.x, aN.y * bN.y) : __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
+  };
+  channel(0, std::true_type{});
+  for (int c = 1; c < nch; ++c) channel(c, std::false_type{});
   // c2r ignores the imaginary parts of the DC and Nyquist bins
   if (lane == 0) y[0].y = 0.f;
 
@@ -312,12 +319,40 @@ struct Best {
   int lag;
 };
 
+// Wave-wide max / min with DPP row operations (max and min are idempotent,
+// so lanes without a DPP source keep their own value); the result is read
+// from lane 63 and is wave-uniform.
+__device__ __forceinline__ float dpp_f(float v, int ctrl_sel) {
+  const int iv = __float_as_int(v);
+  int r = iv;
+  switch (ctrl_sel) {
+    case 0: r = __builtin_amdgcn_update_dpp(iv, iv, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]
+    case 1: r = __builtin_amdgcn_update_dpp(iv, iv, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
+    case 2: r = __builtin_amdgcn_update_dpp(iv, iv, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
+    case 3: r = __builtin_amdgcn_update_dpp(iv, iv, 0x140, 0xF, 0xF, false); break;  // row_mirror
+    case 4: r = __builtin_amdgcn_update_dpp(iv, iv, 0x142, 0xF, 0xF, false); break;  // row_bcast:15
+    default: r = __builtin_amdgcn_update_dpp(iv, iv, 0x143, 0xF, 0xF, false); break; // row_bcast:31
+  }
+  return __int_as_float(r);
+}
+__device__ __forceinline__ void wave_maxmin(float& mx, float& mn) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    mx = fmaxf(mx, dpp_f(mx, k));
+    mn = fminf(mn, dpp_f(mn, k));
+  }
+  mx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), 63));
+  mn = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mn), 63));
+}
+
 // Reference scan rule over the wave's values: strict > (<) starting from
-// (0, lag 0), first occurrence.  Pass 1 finds the exact extrema (max3/min3);
-// pass 2 compares every value with extremum -/+ tol: the ballot counts are
-// summed on the scalar unit, and the last slot whose ballot is non-zero
-// (with its lane mask) locates the extremum.  More than one value within tol
-// flags a near tie, which the FP64 resolver settles.
+// (0, lag 0), first occurrence.  Pass 1 finds the exact extrema (max3/min3,
+// DPP reduction); pass 2 ballots, per register, whether max(x, y) (min(x, y))
+// reaches extremum -/+ tol, and only for the rare registers where one does
+// ballots x and y separately: the ballot counts are summed on the scalar
+// unit, and the last slot whose ballot is non-zero (with its lane mask)
+// locates the extremum.  More than one value within tol flags a near tie,
+// which the FP64 resolver settles.
 template <int R>
 __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol, float thr,
                                           Best& pos, Best& neg, bool& tie_p, bool& tie_n) {
@@ -327,28 +362,32 @@ __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol,
     mx = fmaxf(mx, fmaxf(y[r].x, y[r].y));
     mn = fminf(mn, fminf(y[r].x, y[r].y));
   });
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    mn = fminf(mn, __shfl_xor(mn, o, 64));
-  }
+  wave_maxmin(mx, mn);
   const bool want_p = mx > thr && mx > 0.f;
   const bool want_n = mn < -thr && mn < 0.f;
   const float cp = mx - tol, cn = mn + tol;
   int cnt_p = 0, cnt_n = 0, slot_p = -1, slot_n = -1;
   unsigned long long msk_p = 0, msk_n = 0;
   if (want_p || want_n) {
-    sfor<0, 2 * R>([&](auto sc) {
-      constexpr int s = decltype(sc)::value;
-      const float v = (s & 1) ? y[s >> 1].y : y[s >> 1].x;
-      const unsigned long long bp = __ballot(v >= cp);
-      const unsigned long long bn = __ballot(v <= cn);
-      cnt_p += __popcll(bp);
-      cnt_n += __popcll(bn);
-      slot_p = bp ? s : slot_p;
-      msk_p = bp ? bp : msk_p;
-      slot_n = bn ? s : slot_n;
-      msk_n = bn ? bn : msk_n;
+    sfor<0, R>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      // recomputed here (the asm keeps the compiler from holding 2R pass-1
+      // maxima live across the reduction)
+      float px, pn;
+      asm("v_max_f32 %0, %1, %2" : "=v"(px) : "v"(y[r].x), "v"(y[r].y));
+      asm("v_min_f32 %0, %1, %2" : "=v"(pn) : "v"(y[r].x), "v"(y[r].y));
+      if (__ballot(px >= cp)) {
+        const unsigned long long bx = __ballot(y[r].x >= cp), by = __ballot(y[r].y >= cp);
+        cnt_p += __popcll(bx) + __popcll(by);
+        if (bx) { slot_p = 2 * r; msk_p = bx; }
+        if (by) { slot_p = 2 * r + 1; msk_p = by; }
+      }
+      if (__ballot(pn <= cn)) {
+        const unsigned long long bx = __ballot(y[r].x <= cn), by = __ballot(y[r].y <= cn);
+        cnt_n += __popcll(bx) + __popcll(by);
+        if (bx) { slot_n = 2 * r; msk_n = bx; }
+        if (by) { slot_n = 2 * r + 1; msk_n = by; }
+      }
     });
   }
   pos.v = want_p ? mx : (mx > 0.f ? mx : 0.f);
@@ -411,7 +450,8 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
     for (int j = (dense ? jb : max(jb, i)) + wv; j < je; j += kWaves) {
       cf y[R];
       const float2* xj = cols + (long long)j * a.n_channels * a.spec_stride;
-      pair_inverse<R, !XI_LDS, CT>(y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
+      pair_inverse<R, !XI_LDS, CT>(y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride,
+                                                             a.n_channels, twl, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
       bool tp, tn;
