@@ -110,11 +110,11 @@ def lag_forms(ev, tr, stream, steps, warmup):
     return out
 
 
-def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed_rate: float):
+def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed_rates):
     """Serving form: a 3-channel 2048-sample catalog resident in HBM; requests
     of 1..256 events with lengths U[512, 4096] at Poisson 200 req/s, top-10
-    matches each; then 64-event requests at a fixed Poisson rate for the
-    per-request latency."""
+    matches each; then 64-event requests at a sweep of Poisson rates for the
+    per-request latency against load."""
     from seismic_amd import serving, synth
     log(f"bench: serving catalog {n_catalog} x 3 x 2048")
     cat, _ = synth.make_events(n_catalog, 3, 2048, seed=777)
@@ -150,8 +150,13 @@ def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed
     log(f"bench: serving stream, {stream_requests} requests at 200 req/s")
     out["stream_1_256_events"] = leg(reqs, 200.0, 12)
     reqs = serving.make_requests(fixed_requests, 3, (512, 4096), (64, 64), seed=13)
-    log(f"bench: serving 64-event requests, {fixed_requests} at {fixed_rate} req/s")
-    out["requests_64_events"] = leg(reqs, fixed_rate, 14)
+    sweep = []
+    for rate in fixed_rates:
+        log(f"bench: serving 64-event requests, {fixed_requests} at {rate} req/s")
+        sweep.append(leg(reqs, rate, 14))
+    out["requests_64_events"] = sweep
+    ok = [r["offered_req_per_s"] for r in sweep if r["p99_ms"] < 50.0]
+    out["p99_below_50ms_up_to_req_per_s"] = max(ok) if ok else None
     srv.close()
     return out
 
@@ -230,7 +235,8 @@ def main():
     ap.add_argument("--serve-catalog", type=int, default=32768)
     ap.add_argument("--serve-stream-requests", type=int, default=64)
     ap.add_argument("--serve-64-requests", type=int, default=60)
-    ap.add_argument("--serve-64-rate", type=float, default=10.0)
+    ap.add_argument("--serve-64-rates", type=str, default="5,10,20",
+                    help="comma-separated Poisson rates (req/s) of the 64-event requests")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (scales PMC byte counters)")
     args = ap.parse_args()
@@ -294,7 +300,7 @@ def main():
     serve = None
     if not args.no_serving:
         serve = serving_leg(args.serve_catalog, args.serve_stream_requests, args.serve_64_requests,
-                            args.serve_64_rate)
+                            [float(r) for r in args.serve_64_rates.split(",") if r])
         log(f"bench: serving {json.dumps(serve)}")
 
     tiles = None

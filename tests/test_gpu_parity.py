@@ -197,3 +197,74 @@ def test_extended_p8192_channel_counts(oracle, C, full):
     sure_n = ref["gap_neg"] >= GPU_EXACT_GAP
     assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
     assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
+
+
+def test_edge_one_and_two_events(oracle):
+    """N=1 (diagonal only) and N=2 through the drop-in."""
+    ev, _ = synth.make_events(2, 1, 1024, seed=71)
+    for n in (1, 2):
+        x = ev[:n, 0]
+        print(f"case edge N={n}", flush=True)
+        vp, lp, vn, ln = _run_dropin(x, 512)
+        padded, rev, L, P = synth.pad_like_binding(x)
+        ref = np_ref.reference_f64(padded, rev, L, 512)
+        assert vp.shape == (n * (n + 1) // 2,)
+        assert np.abs(vp - ref["pos"]).max() <= COEF_TOL and np.abs(vn - ref["neg"]).max() <= COEF_TOL
+        assert abs(vp[0] - 1.0) < 1e-5 and lp[0] == 0   # autocorrelation peak at lag 0
+
+
+def test_edge_zero_energy_events():
+    """All-zero traces give (0, 0) with every partner; other pairs unchanged."""
+    from seismic_amd import engine
+    n, C, L, P = 12, 2, 1000, 2048
+    ev, _ = synth.make_events(n, C, L, seed=72)
+    ev0 = ev.copy()
+    ev0[3] = 0.0
+    ev0[7, 1] = 0.0     # one silent channel: that channel contributes 0
+    a = engine.allpairs_host(ev0, P, -P // 2 + 1, P // 2)
+    iu = np.triu_indices(n)
+    for k, (i, j) in enumerate(zip(*iu)):
+        if i == 3 or j == 3:
+            assert a[0][k] == 0 and a[1][k] == 0 and a[2][k] == 0 and a[3][k] == 0
+    # channel 0 of event 7 unchanged: the pair (7, 7) peaks at 1/2 (one of two channels)
+    k77 = 7 * n - 7 * 6 // 2
+    assert abs(a[0][k77] - 0.5) < 1e-5 and a[1][k77] == 0
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_edge_single_lag_window(oracle, form):
+    """lag_lo == lag_hi: the value at that one lag (or 0 when not beyond 0)."""
+    from seismic_amd import engine
+    n, L, lag = 40, 600, 17
+    ev, _ = synth.make_events(n, 1, L, seed=73)
+    P = 2048 if form == 0 else 0
+    print(f"case single lag form={form}", flush=True)
+    vp, lp, vn, ln = engine.allpairs_host(ev, P, lag, lag, form=form)
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, lag, lag, P=P if P else 0, circular=form == 0)
+    tol = 2e-3 if form == 2 else 1e-5
+    assert np.abs(vp - ref["pos"]).max() <= tol and np.abs(vn - ref["neg"]).max() <= tol
+    assert set(np.unique(lp)) <= {0, lag} and set(np.unique(ln)) <= {0, lag}
+    clear_p = ref["pos"] > 1e-4          # value at the lag clearly positive
+    clear_n = ref["neg"] < -1e-4         # clearly negative
+    assert (lp[clear_p] == lag).all() and (lp[clear_n] == 0).all()
+    assert (ln[clear_n] == lag).all() and (ln[clear_p] == 0).all()
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_edge_eight_channels_ragged(oracle, form):
+    """C=8 (the maximum; the FFT form reads rows from global memory) and a
+    ragged N=45 (not a multiple of the 32-event tile)."""
+    from seismic_amd import engine
+    n, C, L = 45, 8, 1024
+    ev, _ = synth.make_events(n, C, L, seed=74)
+    P = 4096 if form == 0 else 0
+    lo, hi = -300, 300
+    print(f"case C=8 form={form}", flush=True)
+    vp, lp, vn, ln = engine.allpairs_host(ev, P, lo, hi, form=form)
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, lo, hi, P=P if P else 0, circular=form == 0)
+    tol, gap = {0: (1e-5, 2e-6), 1: (1e-5, 4e-5), 2: (2e-3, 1e-3)}[form]
+    assert np.abs(vp - ref["pos"]).max() <= tol and np.abs(vn - ref["neg"]).max() <= tol
+    sure = ref["gap_pos"] >= gap
+    assert np.array_equal(lp[sure], ref["lag_pos"][sure])
