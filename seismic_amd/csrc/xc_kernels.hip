@@ -346,21 +346,28 @@ __device__ __forceinline__ void wave_maxmin(float& mx, float& mn) {
 }
 
 // Reference scan rule over the wave's values: strict > (<) starting from
-// (0, lag 0), first occurrence.  Pass 1 finds the exact extrema (max3/min3,
-// DPP reduction); pass 2 ballots, per register, whether max(x, y) (min(x, y))
-// reaches extremum -/+ tol, and only for the rare registers where one does
-// ballots x and y separately: the ballot counts are summed on the scalar
-// unit, and the last slot whose ballot is non-zero (with its lane mask)
-// locates the extremum.  More than one value within tol flags a near tie,
-// which the FP64 resolver settles.
+// (0, lag 0), first occurrence.  Pass 1 keeps the max (min) of each group of
+// two registers (4 values) and reduces them to the exact extrema (DPP wave
+// reduction); pass 2 ballots each group maximum (minimum) against extremum
+// -/+ tol, and only for the rare groups where that ballot is non-zero
+// ballots the group's four values separately: the ballot counts are summed on
+// the scalar unit, and the last slot whose ballot is non-zero (with its lane
+// mask) locates the extremum.  More than one value within tol flags a near
+// tie, which the FP64 resolver settles.
 template <int R>
 __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol, float thr,
                                           Best& pos, Best& neg, bool& tie_p, bool& tie_n) {
+  static_assert(R % 2 == 0, "groups of two registers");
+  constexpr int G = R / 2;
+  float gp[G], gn[G];
   float mx = 0.f, mn = 0.f;
-  sfor<0, R>([&](auto rc) {
-    constexpr int r = decltype(rc)::value;
-    mx = fmaxf(mx, fmaxf(y[r].x, y[r].y));
-    mn = fminf(mn, fminf(y[r].x, y[r].y));
+  sfor<0, G>([&](auto gc) {
+    constexpr int g = decltype(gc)::value;
+    const cf a = y[2 * g], b = y[2 * g + 1];
+    gp[g] = fmaxf(fmaxf(a.x, a.y), fmaxf(b.x, b.y));
+    gn[g] = fminf(fminf(a.x, a.y), fminf(b.x, b.y));
+    mx = fmaxf(mx, gp[g]);
+    mn = fminf(mn, gn[g]);
   });
   wave_maxmin(mx, mn);
   const bool want_p = mx > thr && mx > 0.f;
@@ -369,24 +376,25 @@ __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol,
   int cnt_p = 0, cnt_n = 0, slot_p = -1, slot_n = -1;
   unsigned long long msk_p = 0, msk_n = 0;
   if (want_p || want_n) {
-    sfor<0, R>([&](auto rc) {
-      constexpr int r = decltype(rc)::value;
-      // recomputed here (the asm keeps the compiler from holding 2R pass-1
-      // maxima live across the reduction)
-      float px, pn;
-      asm("v_max_f32 %0, %1, %2" : "=v"(px) : "v"(y[r].x), "v"(y[r].y));
-      asm("v_min_f32 %0, %1, %2" : "=v"(pn) : "v"(y[r].x), "v"(y[r].y));
-      if (__ballot(px >= cp)) {
-        const unsigned long long bx = __ballot(y[r].x >= cp), by = __ballot(y[r].y >= cp);
-        cnt_p += __popcll(bx) + __popcll(by);
-        if (bx) { slot_p = 2 * r; msk_p = bx; }
-        if (by) { slot_p = 2 * r + 1; msk_p = by; }
+    sfor<0, G>([&](auto gc) {
+      constexpr int g = decltype(gc)::value;
+      if (__ballot(gp[g] >= cp)) {
+        sfor<0, 4>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          const float v = (q & 1) ? y[2 * g + (q >> 1)].y : y[2 * g + (q >> 1)].x;
+          const unsigned long long b = __ballot(v >= cp);
+          cnt_p += __popcll(b);
+          if (b) { slot_p = 4 * g + q; msk_p = b; }
+        });
       }
-      if (__ballot(pn <= cn)) {
-        const unsigned long long bx = __ballot(y[r].x <= cn), by = __ballot(y[r].y <= cn);
-        cnt_n += __popcll(bx) + __popcll(by);
-        if (bx) { slot_n = 2 * r; msk_n = bx; }
-        if (by) { slot_n = 2 * r + 1; msk_n = by; }
+      if (__ballot(gn[g] <= cn)) {
+        sfor<0, 4>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          const float v = (q & 1) ? y[2 * g + (q >> 1)].y : y[2 * g + (q >> 1)].x;
+          const unsigned long long b = __ballot(v <= cn);
+          cnt_n += __popcll(b);
+          if (b) { slot_n = 4 * g + q; msk_n = b; }
+        });
       }
     });
   }
