@@ -5,7 +5,9 @@ from __future__ import annotations
 import ctypes
 import os
 
-LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+# SEISMIC_AMD_LIB_DIR selects another build of the libraries (e.g. a variant
+# built with SEISMIC_AMD_BUILD_DIR) without touching the in-tree one.
+LIB_DIR = os.environ.get("SEISMIC_AMD_LIB_DIR") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_float_p = ctypes.POINTER(ctypes.c_float)

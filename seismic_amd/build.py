@@ -19,8 +19,10 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-LIB = os.path.join(HERE, "lib")
-OBJ = os.path.join(HERE, "_build")
+# SEISMIC_AMD_BUILD_DIR builds into <dir>/lib and <dir>/obj instead of the tree
+_ALT = os.environ.get("SEISMIC_AMD_BUILD_DIR")
+LIB = os.path.join(_ALT, "lib") if _ALT else os.path.join(HERE, "lib")
+OBJ = os.path.join(_ALT, "obj") if _ALT else os.path.join(HERE, "_build")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CXX = os.path.join(ROCM, "llvm", "bin", "clang++")
