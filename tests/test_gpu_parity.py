@@ -268,3 +268,24 @@ def test_edge_eight_channels_ragged(oracle, form):
     assert np.abs(vp - ref["pos"]).max() <= tol and np.abs(vn - ref["neg"]).max() <= tol
     sure = ref["gap_pos"] >= gap
     assert np.array_equal(lp[sure], ref["lag_pos"][sure])
+
+
+def test_dropin_concurrent_host_threads():
+    """The drop-in call is reentrant: four host threads at once get the same
+    results as one at a time."""
+    import threading
+    evs = [synth.make_events(40, 1, 1500, seed=90 + t)[0][:, 0] for t in range(4)]
+    alone = [_run_dropin(e, 400, 0.1) for e in evs]
+    got = [None] * 4
+
+    def work(t):
+        got[t] = _run_dropin(evs[t], 400, 0.1)
+    print("case drop-in 4 threads", flush=True)
+    th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for a, b in zip(alone, got):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)

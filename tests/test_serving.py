@@ -125,3 +125,29 @@ def test_memory_budget_limits_slots():
     with pytest.raises(RuntimeError, match="length"):
         srv.submit(np.zeros((1, 1, 300), np.float32), np.array([300], np.int32))
     srv.close()
+
+
+@pytest.mark.gpu
+def test_concurrent_submitters():
+    """Several host threads submit and wait at once; every result equals the
+    same request served alone."""
+    import threading
+    cat, _ = synth.make_events(200, 2, 800, seed=51)
+    srv = serving.Server(cat, max_query_len=1500, top_k=4, max_batch_events=48, n_slots=3)
+    reqs = serving.make_requests(16, 2, (100, 1500), (1, 24), seed=52, pool_events=32)
+    alone = [srv.wait(srv.submit(e, ln))[:3] for e, ln in reqs]
+    got = [None] * len(reqs)
+
+    def client(k0):
+        for k in range(k0, len(reqs), 4):
+            e, ln = reqs[k]
+            got[k] = srv.wait(srv.submit(e, ln))[:3]
+    th = [threading.Thread(target=client, args=(c,)) for c in range(4)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    for a, b in zip(alone, got):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
+    srv.close()
