@@ -240,38 +240,49 @@ __device__ __forceinline__ void wave_fft(cf (&d)[R], const float2* tw, float2* l
   constexpr int kW64 = CT ? TC::kW64 : TL::kW64;
 #pragma unroll
   for (int k = 1; k < 8; ++k) w64[k] = f2c(tw[kW64 + k * 8 + t1]);
-  // transpose 1 + radix-8 over t2, chunk by chunk
+  // transpose 1 + radix-8 over t2, chunk by chunk.  Software pipelined: the
+  // next chunk's writes are issued right after this chunk's reads (a wave's
+  // LDS operations execute in issue order, so the reads see the old chunk),
+  // and their latency overlaps this chunk's radix-8 DFT.
+  wave_lds_fence();
+#pragma unroll
+  for (int gp = 0; gp < 8; ++gp) lds[gp * 72 + lane] = c2f(d[gp]);
+  wave_lds_fence();
   sfor<0, R / 8>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
-    wave_lds_fence();
-#pragma unroll
-    for (int gp = 0; gp < 8; ++gp) lds[gp * 72 + lane] = c2f(d[8 * s + gp]);
-    wave_lds_fence();
     cf e[8];
 #pragma unroll
     for (int t2 = 0; t2 < 8; ++t2) e[t2] = f2c(lds[g * 72 + t1 + 8 * t2]);
+    if constexpr (s + 1 < R / 8) {
+#pragma unroll
+      for (int gp = 0; gp < 8; ++gp) lds[gp * 72 + lane] = c2f(d[8 * (s + 1) + gp]);
+    }
     reg_fft<8, SIGN>(e);
 #pragma unroll
     for (int k = 1; k < 8; ++k) e[k] = cmul(e[k], w64[k]);
 #pragma unroll
     for (int k = 0; k < 8; ++k) d[8 * s + k] = e[k];
+    wave_lds_fence();
   });
-  // transpose 2 + radix-8 over t1
+  // transpose 2 + radix-8 over t1, pipelined the same way
   const int h = t1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) lds[g * 72 + k * 9 + t1] = c2f(d[k]);
+  wave_lds_fence();
   sfor<0, R / 8>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
-    wave_lds_fence();
-#pragma unroll
-    for (int k = 0; k < 8; ++k) lds[g * 72 + k * 9 + t1] = c2f(d[8 * s + k]);
-    wave_lds_fence();
     cf e[8];
 #pragma unroll
     for (int tp = 0; tp < 8; ++tp) e[tp] = f2c(lds[g * 72 + h * 9 + tp]);
+    if constexpr (s + 1 < R / 8) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) lds[g * 72 + k * 9 + t1] = c2f(d[8 * (s + 1) + k]);
+    }
     reg_fft<8, SIGN>(e);
 #pragma unroll
     for (int k = 0; k < 8; ++k) d[8 * s + k] = e[k];
+    wave_lds_fence();
   });
-  wave_lds_fence();
 }
 
 // Frequency index held by (lane, register 8s+k) after wave_fft.
