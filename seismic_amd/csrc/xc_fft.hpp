@@ -134,6 +134,38 @@ __device__ __forceinline__ cf twmul(cf b) {
   }
 }
 
+// Radix-2 butterfly (a, b) -> (a + w b, a - w b), w = W_len^K.  Trivial and
+// W_8-type twiddles use twmul (the compiler fuses those into 6 operations);
+// general ones use the tangent form (Linzer-Feig): w b = c (b.x - r b.y,
+// b.y + r b.x) with r = s/c (or the cotangent form, scaled by s, when
+// |s| > |c|), 6 FMAs instead of 4 multiply-adds plus 4 additions.
+template <int K, int LEN, int SIGN>
+__device__ __forceinline__ void bfly(cf& a, cf& b) {
+  constexpr int k = K % LEN;
+  constexpr bool trivial = k == 0 || 4 * k == LEN || 4 * k == 2 * LEN || 4 * k == 3 * LEN ||
+                           8 * k == LEN || 8 * k == 3 * LEN || 8 * k == 5 * LEN || 8 * k == 7 * LEN;
+  if constexpr (trivial) {
+    const cf t = twmul<K, LEN, SIGN>(b);
+    b = csub(a, t);
+    a = cadd(a, t);
+  } else {
+    constexpr cxd w = cx_twiddle(k, LEN, SIGN);
+    constexpr bool tan_form = (w.re < 0 ? -w.re : w.re) >= (w.im < 0 ? -w.im : w.im);
+    constexpr float sc = tan_form ? (float)w.re : (float)w.im;        // common scale
+    constexpr float r = tan_form ? (float)(w.im / w.re) : (float)(w.re / w.im);
+    float ux, uy;
+    if constexpr (tan_form) {
+      ux = __builtin_fmaf(-r, b.y, b.x);
+      uy = __builtin_fmaf(r, b.x, b.y);
+    } else {
+      ux = __builtin_fmaf(r, b.x, -b.y);
+      uy = __builtin_fmaf(r, b.y, b.x);
+    }
+    b = cf{__builtin_fmaf(-sc, ux, a.x), __builtin_fmaf(-sc, uy, a.y)};
+    a = cf{__builtin_fmaf(sc, ux, a.x), __builtin_fmaf(sc, uy, a.y)};
+  }
+}
+
 // In-place length-N DFT of a register array, natural order in and out.
 template <int N, int SIGN>
 __device__ __forceinline__ void reg_fft(cf (&a)[N]) {
@@ -151,9 +183,7 @@ __device__ __forceinline__ void reg_fft(cf (&a)[N]) {
         constexpr int k = decltype(kk)::value;
         constexpr int ia = decltype(blk)::value * len + k;
         constexpr int ib = ia + half;
-        cf t = twmul<k, len, SIGN>(b[ib]);
-        b[ib] = csub(b[ia], t);
-        b[ia] = cadd(b[ia], t);
+        bfly<k, len, SIGN>(b[ia], b[ib]);
       });
     });
   });
