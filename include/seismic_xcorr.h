@@ -169,6 +169,8 @@ int sxc_server_submit(sxc_server* server, const float* events, long long stride,
 int sxc_server_wait(sxc_server* server, long long request_id, sxc_match* out, double* latency_ms);
 int sxc_server_stats(sxc_server* server, sxc_serve_stats* out);
 int sxc_server_reset_stats(sxc_server* server);
+/* Serves every request still queued, then frees the server.  No other thread
+ * may be inside a call on this server (e.g. sxc_server_wait) meanwhile. */
 int sxc_server_destroy(sxc_server* server);
 
 /* ------------------------------------------------------------------------
@@ -207,6 +209,7 @@ int sxc_tiled_next(sxc_tiled* t, sxc_block_info* info, unsigned short* coef_pos,
                    unsigned short* coef_neg, short* lag_neg, long long ld);
 /* pairs delivered so far (upper triangle incl. diagonal), near ties, overflow */
 int sxc_tiled_stats(sxc_tiled* t, sxc_stats* out);
+/* Waits for blocks still in flight, then frees the run. */
 int sxc_tiled_destroy(sxc_tiled* t);
 
 /* Host-memory convenience wrapper: copies traces in, runs, copies the packed
