@@ -311,7 +311,7 @@ static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_
   ra.lpos = lp;
   ra.vneg = vn;
   ra.lneg = ln;
-  HIP_CHECK(launch_lag_resolve(ra, 2048, st));
+  HIP_CHECK(launch_lag_resolve(ra, 8192, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
 }

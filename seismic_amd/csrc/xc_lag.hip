@@ -339,10 +339,22 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, const float* __restrict__ tj,
                                                int L, int lag, int lane) {
-  double s = 0.0;
+  // four samples per lane per trip, loads issued together, four accumulators
   const int n0 = lag > 0 ? lag : 0;
   const int n1 = lag < 0 ? L + lag : L;
-  for (int n = n0 + lane; n < n1; n += 64) s += (double)ti[n] * (double)tj[n - lag];
+  auto term = [&](int n) -> double {
+    const bool ok = n < n1;
+    const float a = ti[ok ? n : n0], b = tj[ok ? n - lag : n0 - lag];
+    return ok ? (double)a * (double)b : 0.0;
+  };
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int n = n0 + lane; n < n1; n += 256) {
+    s0 += term(n);
+    s1 += term(n + 64);
+    s2 += term(n + 128);
+    s3 += term(n + 192);
+  }
+  double s = (s0 + s1) + (s2 + s3);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   return s;

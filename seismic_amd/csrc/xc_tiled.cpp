@@ -149,7 +149,7 @@ int enqueue_block(sxc_tiled* t, int b) {
   ra.out_ld = t->block;
   ra.out_row0 = row0;
   ra.out_col0 = col0;
-  HIP_CHECK(launch_lag_resolve(ra, 2048, st));
+  HIP_CHECK(launch_lag_resolve(ra, 8192, st));
   HIP_CHECK(hipMemcpyAsync(t->h_ties + slot, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   // slot reuse: block b - 2's copy out of this slot must be complete
   if (b >= 2) HIP_CHECK(hipStreamWaitEvent(st, t->e_copied[slot], 0));

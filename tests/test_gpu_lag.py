@@ -44,15 +44,23 @@ def test_lag_form_vs_f64(oracle, name, n, C, L, lo, hi, thr):
     assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
 
 
-def test_lag_matches_fft_form():
+def test_lag_matches_fft_form(oracle):
     """With P >= L + max|lag| the FFT form's circular lags equal linear lags:
-    the two forms must agree (values to 1e-5, lags off near ties)."""
+    the two forms must agree (values to 1e-5; lags on every pair whose top two
+    FP64 values are not an exact tie -- e.g. the diagonal pairs' symmetric
+    autocorrelation has equal values at -l and +l, where the summation order
+    of each form's FP64 re-evaluation decides)."""
     n, C, L, P = 48, 3, 2048, 4096
     ev, _ = synth.make_events(n, C, L, seed=5)
     a = engine.allpairs_host(ev, P, -512, 512)
     b = engine.allpairs_host(ev, 0, -512, 512, form=engine.FORM_LAG_BF16X3)
     assert np.abs(a[0] - b[0]).max() < 2e-5 and np.abs(a[2] - b[2]).max() < 2e-5
-    assert (a[1] == b[1]).mean() > 0.999 and (a[3] == b[3]).mean() > 0.999
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, -512, 512, circular=False)
+    sp = ref["gap_pos"] > 1e-9
+    sn = ref["gap_neg"] > 1e-9
+    assert np.array_equal(a[1][sp], b[1][sp]) and np.array_equal(a[3][sn], b[3][sn])
+    assert (a[1] == b[1]).mean() > 0.99 and (a[3] == b[3]).mean() > 0.99
 
 
 def test_lag_exact_shift_duplicates():
