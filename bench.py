@@ -234,7 +234,7 @@ def main():
     ap.add_argument("--tiled-block", type=int, default=8192)
     ap.add_argument("--serve-catalog", type=int, default=32768)
     ap.add_argument("--serve-stream-requests", type=int, default=64)
-    ap.add_argument("--serve-64-requests", type=int, default=60)
+    ap.add_argument("--serve-64-requests", type=int, default=100)
     ap.add_argument("--serve-64-rates", type=str, default="5,10,20",
                     help="comma-separated Poisson rates (req/s) of the 64-event requests")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -657,40 +657,55 @@ hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream
   return hipGetLastError();
 }
 
-// One workgroup (256 threads) per row: each thread keeps a sorted top-k of a
-// strided share of the columns in registers, then wave 0 merges the 256
-// candidate lists with k rounds of a wave-wide argmax.
+// One workgroup (256 threads) per row: each thread keeps a sorted list of its
+// K best (K >= k, a compile-time capacity) over a strided share of the
+// columns in registers -- a candidate not better than the thread's K-th best
+// is rejected with one comparison -- then wave 0 merges the 256 lists with k
+// rounds of a wave-wide argmax.  Ties go to the smaller column.
 constexpr int kTopkMax = 32;
 __device__ __forceinline__ bool better(float v, int c, float bv, int bc) {
   return v > bv || (v == bv && c < bc);
 }
+template <int K>
 __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const int* __restrict__ l,
                                               int n_cols, int k, float* out_v, int* out_l, int* out_idx) {
-  __shared__ float cv[256 * kTopkMax];
-  __shared__ int cc[256 * kTopkMax];
+  __shared__ float cv[256 * K];
+  __shared__ int cc[256 * K];
   const int row = blockIdx.x;
   const float* rv = v + (long long)row * n_cols;
-  float tv[kTopkMax];
-  int tc[kTopkMax];
+  float tv[K];
+  int tc[K];
 #pragma unroll
-  for (int q = 0; q < kTopkMax; ++q) { tv[q] = -INFINITY; tc[q] = 0x7fffffff; }
-  for (int c = threadIdx.x; c < n_cols; c += 256) {
-    float x = rv[c];
-    int xc = c;
+  for (int q = 0; q < K; ++q) { tv[q] = -INFINITY; tc[q] = 0x7fffffff; }
+  // eight loads in flight per thread, then the eight candidates in order
+  constexpr int kBatch = 8;
+  for (int c0 = threadIdx.x; c0 < n_cols; c0 += 256 * kBatch) {
+    float xb[kBatch];
 #pragma unroll
-    for (int q = 0; q < kTopkMax; ++q) {  // insertion, keeps tv sorted descending
-      if (q < k && better(x, xc, tv[q], tc[q])) {
-        const float sv = tv[q];
-        const int sc = tc[q];
-        tv[q] = x; tc[q] = xc;
-        x = sv; xc = sc;
+    for (int u = 0; u < kBatch; ++u) {
+      const int c = c0 + 256 * u;
+      xb[u] = c < n_cols ? rv[c] : -INFINITY;
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      float x = xb[u];
+      int xc = c0 + 256 * u;
+      if (xc >= n_cols || !better(x, xc, tv[K - 1], tc[K - 1])) continue;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {  // insertion, keeps tv sorted descending
+        if (better(x, xc, tv[q], tc[q])) {
+          const float sv = tv[q];
+          const int sc = tc[q];
+          tv[q] = x; tc[q] = xc;
+          x = sv; xc = sc;
+        }
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < kTopkMax; ++q) {
-    cv[threadIdx.x * kTopkMax + q] = q < k ? tv[q] : -INFINITY;
-    cc[threadIdx.x * kTopkMax + q] = q < k ? tc[q] : 0x7fffffff;
+  for (int q = 0; q < K; ++q) {
+    cv[threadIdx.x * K + q] = tv[q];
+    cc[threadIdx.x * K + q] = tc[q];
   }
   __syncthreads();
   if (threadIdx.x >= 64) return;
@@ -702,9 +717,9 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int t = lane + 64 * u;
-      if (head[u] < k) {
-        const float x = cv[t * kTopkMax + head[u]];
-        const int xc = cc[t * kTopkMax + head[u]];
+      if (head[u] < K) {
+        const float x = cv[t * K + head[u]];
+        const int xc = cc[t * K + head[u]];
         if (better(x, xc, bv, bc)) { bv = x; bc = xc; bl = u; }
       }
     }
@@ -732,7 +747,12 @@ hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int
                        int* out_l, int* out_idx, hipStream_t st) {
   if (n_rows <= 0) return hipSuccess;
   if (k < 1 || k > kTopkMax) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_topk, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
+  if (k <= 8)
+    hipLaunchKernelGGL(k_topk<8>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
+  else if (k <= 16)
+    hipLaunchKernelGGL(k_topk<16>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
+  else
+    hipLaunchKernelGGL(k_topk<32>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
   return hipGetLastError();
 }
 
