@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the all-pairs normalized cross-correlation on one MI355X.
 
-Workload (the headline configuration): N=4096 events x 3 channels x 2048
+Workload: N=4096 events x 3 channels x 2048
 float32 samples, seeded noise + Ricker events, FFT period P=4096, full lag
 range (-2047..2048), channel coefficients averaged per lag.  One step = one
 pass of the hot path over the resident batch: batched R2C of all 12288

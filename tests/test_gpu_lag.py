@@ -78,7 +78,7 @@ def test_lag_exact_shift_duplicates():
         assert abs(vp[1] - 1) < tol and abs(vp[2] - 1) < tol and abs(vp[0] - 1) < tol
 
 
-def test_cfg3_forms_vs_f64_sampled(oracle):
+def test_n8192_l4096_w512_forms_vs_f64_sampled(oracle):
     """Precision configuration at full size: N=8192, 1 channel, L=4096,
     window +-512, all 33.5M pairs in bf16x3, fp16 and the FP32-FFT form
     (P=8192, circular lags equal the linear ones since P >= L + 512);
@@ -97,12 +97,12 @@ def test_cfg3_forms_vs_f64_sampled(oracle):
     for name, form, P, coef_tol, gap in (("bf16x3", engine.FORM_LAG_BF16X3, 0, 1e-5, 4e-5),
                                          ("fp16", engine.FORM_LAG_FP16, 0, 2e-3, 1e-3),
                                          ("fft_f32", engine.FORM_FFT_F32, 8192, 1e-5, 2e-6)):
-        print(f"case cfg3 {name}", flush=True)
+        print(f"case N=8192 L=4096 +-512 {name}", flush=True)
         vp, lp, vn, ln = engine.allpairs_host(ev, P, -w, w, form=form)
         dp = np.abs(vp[idx] - ref["pos"]).max()
         dn = np.abs(vn[idx] - ref["neg"]).max()
         agree = (lp[idx][keep] == ref["lag_pos"][keep]).mean()
-        print(f"cfg3 {name}: max |dcoef| pos {dp:.3e} neg {dn:.3e}; lag agreement {agree:.6f} "
+        print(f"N=8192 L=4096 +-512 {name}: max |dcoef| pos {dp:.3e} neg {dn:.3e}; lag agreement {agree:.6f} "
               f"on {int(keep.sum())} sampled pairs")
         assert dp <= coef_tol and dn <= coef_tol
         sure = keep & (ref["gap_pos"] >= gap)

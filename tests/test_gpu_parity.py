@@ -47,17 +47,17 @@ def _check(gpu, ref, oracle_lags=None, name=""):
     return dp, dn
 
 
-def test_cfg1_dropin_full_window(oracle):
+def test_n256_l1024_dropin_full_window(oracle):
     """N=256, 1 channel, L=1024 (P=2048), full lag range (shift 1024)."""
     ev, _ = synth.make_events(256, 1, 1024, seed=101)
     x = ev[:, 0]
-    print("case cfg1 N=256 L=1024 shift=1024", flush=True)
+    print("case N=256 L=1024 shift=1024", flush=True)
     gpu = _run_dropin(x, 1024)
     padded, rev, L, P = synth.pad_like_binding(x)
     ref = np_ref.reference_f64(padded, rev, L, 1024)
     ovp, olp, ovn, oln, _, _ = oracle.correlation_cpp(padded, rev, L, 1024)
-    dp, dn = _check(gpu, ref, (olp, oln), "cfg1")
-    print(f"cfg1 max|dcoef| pos {dp:.2e} neg {dn:.2e}")
+    dp, dn = _check(gpu, ref, (olp, oln), "N=256 L=1024")
+    print(f"N=256 L=1024 max|dcoef| pos {dp:.2e} neg {dn:.2e}")
 
 
 @pytest.mark.parametrize("L,shift,thr", [(1024, 256, 0.2), (1500, 548, 0.0), (700, 300, 0.1),

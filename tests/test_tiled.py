@@ -1,7 +1,7 @@
 """Tiled N x N runs: argument checks on the CPU; on the GPU, equality with
 the packed-triangle execute of the same form (values rounded to float16),
 mirrored blocks, ragged edges, and exact lags of time-shifted duplicates
-at the cfg5 trace shape (L=4096, window +-1024)."""
+at the largest-run trace shape (L=4096, window +-1024)."""
 import numpy as np
 import pytest
 
@@ -82,12 +82,12 @@ def test_tiled_equals_packed(n, C, L, lo, hi, form, block, neg):
 
 
 @pytest.mark.gpu
-def test_tiled_cfg5_shape_duplicates(oracle):
+def test_tiled_l4096_w1024_duplicates(oracle):
     """L=4096, window +-1024, bf16x3, log-uniform SNR in [0.3, 30] and 10%
     zero-filled time-shifted duplicates, N=640 in blocks of 256 (ragged)."""
     n, L, w = 640, 4096, 1024
     ev, meta = synth.make_events(n, 1, L, seed=65536, snr=(0.3, 30.0), log_snr=True, dup_frac=0.1)
-    print(f"case tiled cfg5-shape n={n} dups={len(meta['dups'])}", flush=True)
+    print(f"case tiled L=4096 +-1024 n={n} dups={len(meta['dups'])}", flush=True)
     coef, lag, _, _, nb, st = _tiled_all(ev, -w, w, engine.FORM_LAG_BF16X3, 256, False)
     assert nb == 6 and st["tie_overflow"] == 0
     iu = np.triu_indices(n)
@@ -119,6 +119,6 @@ def test_tiled_cfg5_shape_duplicates(oracle):
     err = np.abs(got - ref["pos"]).max()
     sure = ref["gap_pos"] >= 4e-5
     agree = (full_lag[pi, pj] == ref["lag_pos"]).mean()
-    print(f"tiled cfg5-shape: max |dcoef| vs FP64 {err:.3e} (float16 output), lag agreement {agree:.5f}")
+    print(f"tiled L=4096 +-1024: max |dcoef| vs FP64 {err:.3e} (float16 output), lag agreement {agree:.5f}")
     assert err <= 1e-3
     assert np.array_equal(full_lag[pi, pj][sure], ref["lag_pos"][sure])
