@@ -151,3 +151,28 @@ def test_concurrent_submitters():
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
     srv.close()
+
+
+@pytest.mark.gpu
+def test_serving_one_channel_demean_top32_length_extremes():
+    """1 channel, FP64 mean removal, top_k = 32 (the maximum) and query
+    lengths 1, catalog_len and max_query_len, against the FP64 oracle."""
+    cat, _ = synth.make_events(60, 1, 300, seed=61)
+    cat = cat + np.float32(2.0)                    # offsets the device must remove
+    srv = serving.Server(cat, max_query_len=700, top_k=32, max_batch_events=8, n_slots=2, demean=True)
+    q, _ = synth.make_events(3, 1, 700, seed=62)
+    q = q + np.float32(-1.5)
+    lens = np.array([1, 300, 700], np.int32)
+    for e in range(3):
+        q[e, :, lens[e]:] = 0
+    v, l, idx, _ = srv.wait(srv.submit(q, lens))
+    srv.close()
+    cat0 = cat.astype(np.float64) - cat.astype(np.float64).mean(-1, keepdims=True)
+    for e in range(3):
+        x = q[e, :, :lens[e]].astype(np.float64)
+        x = x - x.mean(-1, keepdims=True)
+        ref = oracle_topk(x[None].astype(np.float32), [lens[e]], cat0.astype(np.float32), 32)[0]
+        rv = np.array([t[0] for t in ref])
+        assert np.abs(v[e] - rv).max() <= 1e-5, e
+        if e == 0:   # a one-sample query has zero energy after mean removal: every match is 0
+            assert np.all(v[e] == 0)
