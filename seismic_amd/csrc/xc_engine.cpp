@@ -209,8 +209,10 @@ int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   p->p0 = std::max(0, (last + 7) / 8 * 8);
   p->ls = (std::max(p->p0 + p->l16, p->p0 + p->l16 - lf + 9) + 7) / 8 * 8;
   p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
+  // default bands: at least twice the largest coefficient error against FP64
+  // measured for the form (bf16x3 4.8e-6, fp16 1.1e-4)
   p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol
-                                   : (desc->form == SXC_FORM_LAG_FP16 ? 2e-4f : 2e-5f);
+                                   : (desc->form == SXC_FORM_LAG_FP16 ? 2.5e-4f : 2e-5f);
   p->tie_cap = (int)std::min<long long>(p->n_pairs, 1LL << 24);
   HIP_CHECK(hipGetDevice(&p->device));
   const long long rows = (long long)desc->n_events * desc->n_channels;
