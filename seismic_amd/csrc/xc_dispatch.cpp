@@ -10,7 +10,9 @@ hipError_t launch_forward(int R, const ForwardArgs& a, hipStream_t st) {
   switch (R) {
     case 8: return launch_forward_r<8>(a, st);
     case 16: return launch_forward_r<16>(a, st);
+    case 24: return launch_forward_r<24>(a, st);
     case 32: return launch_forward_r<32>(a, st);
+    case 48: return launch_forward_r<48>(a, st);
     case 64: return launch_forward_r<64>(a, st);
   }
   return hipErrorInvalidValue;
@@ -21,7 +23,9 @@ hipError_t launch_pairs(int R, bool full, const PairArgs& a, int grid, hipStream
   switch (R) {
     case 8: return launch_pairs_r<8>(full, a, grid, st);
     case 16: return launch_pairs_r<16>(full, a, grid, st);
+    case 24: return launch_pairs_r<24>(full, a, grid, st);
     case 32: return launch_pairs_r<32>(full, a, grid, st);
+    case 48: return launch_pairs_r<48>(full, a, grid, st);
     case 64: return launch_pairs_r<64>(full, a, grid, st);
   }
   return hipErrorInvalidValue;
@@ -31,7 +35,9 @@ hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st)
   switch (R) {
     case 8: return launch_resolve_r<8>(a, grid, st);
     case 16: return launch_resolve_r<16>(a, grid, st);
+    case 24: return launch_resolve_r<24>(a, grid, st);
     case 32: return launch_resolve_r<32>(a, grid, st);
+    case 48: return launch_resolve_r<48>(a, grid, st);
     case 64: return launch_resolve_r<64>(a, grid, st);
   }
   return hipErrorInvalidValue;

@@ -79,8 +79,8 @@ int validate_desc(const sxc_plan_desc& d) {
     if (!(d.threshold == d.threshold)) { set_error("threshold is NaN"); return SXC_ERR_INVALID; }
     return SXC_OK;
   }
-  if (P < 1024 || P > 8192 || (P & (P - 1))) {
-    set_error("padded_size must be a power of two in [1024, 8192] (got %d)", P);
+  if (!fft_period_supported(P)) {
+    set_error("padded_size must be one of 1024, 2048, 3072, 4096, 6144, 8192 (got %d)", P);
     return SXC_ERR_INVALID;
   }
   if (d.trace_len < 1 || d.trace_len > P) { set_error("trace_len must be in [1, padded_size] (got %d)", d.trace_len); return SXC_ERR_INVALID; }

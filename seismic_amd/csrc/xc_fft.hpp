@@ -3,7 +3,8 @@
 // A complex FFT of length M = 64*R is held by one wavefront, R values per
 // lane, and computed as a four-step decomposition M = R x 64:
 //   1. lane t holds x[t + 64*m], m < R; a length-R DFT over m runs in
-//      registers (radix-2 DIT, compile-time twiddles, trivial ones skipped);
+//      registers (radix-2 DIT, one radix-3 stage first when R = 3 * 2^k;
+//      compile-time twiddles, trivial ones skipped);
 //   2. twiddle by W_M^(t*k2);
 //   3. the remaining 64-point DFTs (one per k2) run across lanes as 8 x 8:
 //      two LDS transposes, each followed by a register radix-8 DFT, with a
@@ -166,9 +167,24 @@ __device__ __forceinline__ void bfly(cf& a, cf& b) {
   }
 }
 
-// In-place length-N DFT of a register array, natural order in and out.
+// Radix-3 butterfly on (b0, b1, b2): outputs b0 + b1 + b2 and
+// b0 + W3^k b1 + W3^2k b2 for k = 1, 2, W3 = exp(SIGN 2 pi i / 3)
+// = -1/2 + SIGN i sqrt(3)/2; the two rotated outputs share
+// m = b0 - (b1 + b2)/2 and differ by -+ SIGN i sqrt(3)/2 (b1 - b2).
+template <int SIGN>
+__device__ __forceinline__ void bfly3(cf b0, cf b1, cf b2, cf& x0, cf& x1, cf& x2) {
+  constexpr float s3 = SIGN * 0.86602540378443864676f;  // SIGN sin(pi/3)
+  const cf t1 = cadd(b1, b2), t2 = csub(b1, b2);
+  const cf m = cmk(__builtin_fmaf(-0.5f, t1.x, b0.x), __builtin_fmaf(-0.5f, t1.y, b0.y));
+  x0 = cadd(b0, t1);
+  x1 = cmk(__builtin_fmaf(-s3, t2.y, m.x), __builtin_fmaf(s3, t2.x, m.y));
+  x2 = cmk(__builtin_fmaf(s3, t2.y, m.x), __builtin_fmaf(-s3, t2.x, m.y));
+}
+
+// In-place length-N DFT, N = 2^k: radix-2 DIT, natural order in and out.
 template <int N, int SIGN>
-__device__ __forceinline__ void reg_fft(cf (&a)[N]) {
+__device__ __forceinline__ void reg_fft_pow2(cf (&a)[N]) {
+  static_assert((N & (N - 1)) == 0, "N must be a power of two");
   constexpr int LG = ilog2c(N);
   cf b[N];
   sfor<0, N>([&](auto i) {
@@ -188,6 +204,35 @@ __device__ __forceinline__ void reg_fft(cf (&a)[N]) {
     });
   });
   sfor<0, N>([&](auto i) { a[decltype(i)::value] = b[decltype(i)::value]; });
+}
+
+// In-place length-N DFT of a register array, natural order in and out.
+// N = 2^k: radix-2 DIT.  N = 3 * 2^k: decimation in time by 3 -- three
+// length-N/3 DFTs of x[3n + r], then X[k + (N/3) q] = sum_r W_N^(r k)
+// W_3^(r q) F_r[k], one twiddled radix-3 butterfly per k.
+template <int N, int SIGN>
+__device__ __forceinline__ void reg_fft(cf (&a)[N]) {
+  if constexpr (N % 3 == 0) {
+    constexpr int Q = N / 3;
+    static_assert((Q & (Q - 1)) == 0, "N must be 2^k or 3 * 2^k");
+    cf f0[Q], f1[Q], f2[Q];
+    sfor<0, Q>([&](auto nc) {
+      constexpr int n = decltype(nc)::value;
+      f0[n] = a[3 * n];
+      f1[n] = a[3 * n + 1];
+      f2[n] = a[3 * n + 2];
+    });
+    reg_fft<Q, SIGN>(f0);
+    reg_fft<Q, SIGN>(f1);
+    reg_fft<Q, SIGN>(f2);
+    sfor<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      bfly3<SIGN>(f0[k], twmul<k, N, SIGN>(f1[k]), twmul<2 * k, N, SIGN>(f2[k]), a[k], a[k + Q],
+                  a[k + 2 * Q]);
+    });
+  } else {
+    reg_fft_pow2<N, SIGN>(a);
+  }
 }
 
 // Keeps the compiler from moving LDS accesses across this point and makes

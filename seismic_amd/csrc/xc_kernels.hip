@@ -26,7 +26,7 @@
 #include "xc_kernels.h"
 
 #ifndef SXC_R
-#error "compile once per R with -DSXC_R=8|16|32|64"
+#error "compile once per R with -DSXC_R=8|16|24|32|48|64"
 #endif
 
 namespace sxc {
@@ -180,8 +180,8 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   float2* out = a.spec + (long long)row * a.spec_stride;
   // X[k] = (Z[k] + conj Z[M-k])/2 - i/2 exp(-2 pi i k/P) (Z[k] - conj Z[M-k])
   for (int k = lane; k <= M; k += 64) {
-    cf z = f2c(zl[k & (M - 1)]);
-    cf zm = f2c(zl[(M - k) & (M - 1)]);
+    cf z = f2c(zl[k < M ? k : 0]);       // Z[M] = Z[0]
+    cf zm = f2c(zl[k > 0 ? M - k : 0]);
     zm.y = -zm.y;
     cf e = cmk(0.5f * (z.x + zm.x), 0.5f * (z.y + zm.y));
     cf dd = csub(z, zm);
@@ -458,8 +458,10 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
     for (int j = (dense ? jb : max(jb, i)) + wv; j < je; j += kWaves) {
       cf y[R];
       const float2* xj = cols + (long long)j * a.n_channels * a.spec_stride;
-      pair_inverse<R, !XI_LDS, CT>(y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride,
-                                                             a.n_channels, twl, wl, lane);
+      // rows from global memory double the loads in flight: smaller groups at
+      // P >= 6144 keep them within the 256-VGPR budget
+      pair_inverse<R, !XI_LDS, CT, (!XI_LDS && R >= 48) ? 16 : 32>(
+          y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
       bool tp, tn;
@@ -609,11 +611,12 @@ template <int R, bool FULL>
 static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
   // 12 waves (3 per SIMD) when the staged row spectra fit the 160 KiB of a
   // CU, else 8; then 8 waves with the compact tables (P >= 4096 only); rows
-  // from global memory when none of these fit.  P = 8192 always runs 8
-  // waves: its 128 spectrum registers per lane need the 256-VGPR budget.
+  // from global memory when none of these fit.  P = 6144 and 8192 always run
+  // 8 waves: their 96 / 128 spectrum registers per lane need the 256-VGPR
+  // budget.
   constexpr size_t kLds = 160 * 1024;
   const int C = a.n_channels;
-  if constexpr (R < 64) {
+  if constexpr (R <= 32) {
     if (pair_lds_bytes(R, C, 12, false) <= kLds) return launch_pairs_rfw<R, FULL, 12, true, false>(a, grid, st);
   }
   if (pair_lds_bytes(R, C, 8, false) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, false>(a, grid, st);

@@ -54,11 +54,19 @@ using Clock = std::chrono::steady_clock;
     }                                                                                            \
   } while (0)
 
+constexpr int kPeriods[] = {1024, 2048, 3072, 4096, 6144, 8192};
+
+// Smallest FFT period whose circular correlation equals the linear one on
+// every overlapping lag -(lc-1) .. lq-1 (P >= lq + lc - 1).  The pair kernel
+// reports slot n as lag n <= P/2 ? n : n - P; P/2 >= lc keeps every negative
+// lag in the upper half of the slots, so it comes out right, and a positive
+// lag above P/2 comes out as lag - P < -(lc-1), where wrap_lag moves it back.
 int period_for(int lq, int lc) {
-  for (int P = 1024; P <= 8192; P *= 2)
-    if (P >= lq + lc - 1 && P / 2 >= std::max(lq - 1, lc)) return P;
+  for (int P : kPeriods)
+    if (P >= lq + lc - 1 && P / 2 >= lc) return P;
   return 0;
 }
+int wrap_lag(int lag, int P, int lc) { return lag < -(lc - 1) ? lag + P : lag; }
 
 // FFT tables of one period P (forward, inverse, C2R packing, post-twiddle)
 struct Tables {
@@ -313,11 +321,14 @@ void worker_loop(sxc_server* S, int slot_index) {
       for (size_t r = 0; r < batch.size(); ++r) {
         Request& q = *batch[r];
         q.result.resize((size_t)q.n * d.top_k);
-        for (int e = 0; e < q.n; ++e)
+        for (int e = 0; e < q.n; ++e) {
+          const int P = period_for(q.lengths[e], d.catalog_len);
           for (int k = 0; k < d.top_k; ++k) {
             const size_t src = (size_t)row_of[r][e] * d.top_k + k;
-            q.result[(size_t)e * d.top_k + k] = sxc_match{s->h_tv[src], s->h_tl[src], s->h_ti[src]};
+            q.result[(size_t)e * d.top_k + k] =
+                sxc_match{s->h_tv[src], wrap_lag(s->h_tl[src], P, d.catalog_len), s->h_ti[src]};
           }
+        }
         q.status = rc;
         q.t_done = now;
         q.done = true;
@@ -407,7 +418,8 @@ int sxc_server_create(const sxc_server_desc* desc, const float* catalog, sxc_ser
     set_error("catalog upload failed");
     return fail(SXC_ERR_HIP);
   }
-  for (int P = pmin; P <= pmax; P *= 2) {
+  for (int P : kPeriods) {
+    if (P < pmin || P > pmax) continue;
     Tables t;
     int rc = tables_create(P, &t);
     S->tables[P] = t;

@@ -199,6 +199,31 @@ def test_extended_p8192_channel_counts(oracle, C, full):
     assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
 
 
+@pytest.mark.parametrize("P,C,full", [(3072, 1, True), (3072, 3, False), (3072, 8, False),
+                                      (6144, 1, True), (6144, 3, False), (6144, 4, False),
+                                      (6144, 5, False)])
+def test_extended_mixed_radix_periods(oracle, P, C, full):
+    """Periods 3 * 2^k (a radix-3 stage in the register DFT), traces longer
+    than P/2 so the circular wrap matters.  P=3072: 12 waves (C <= 6), 8
+    waves (C=8); P=6144: 8 waves with full tables (C <= 3), compact tables
+    (C=4), rows from global memory (C=5)."""
+    from seismic_amd import engine
+    n, L = 12, P - 200
+    ev, _ = synth.make_events(n, C, L, seed=90 + C + P // 1024)
+    lo, hi = (-P // 2 + 1, P // 2) if full else (-1000, 1400)
+    print(f"case P={P} C={C} L={L} window [{lo}, {hi}]", flush=True)
+    vp, lp, vn, ln = engine.allpairs_host(ev, P, lo, hi)
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, lo, hi, P=P, circular=True)
+    dp, dn = np.abs(vp - ref["pos"]).max(), np.abs(vn - ref["neg"]).max()
+    print(f"P={P} C={C}: max |dcoef| pos {dp:.3e} neg {dn:.3e}")
+    assert dp <= COEF_TOL and dn <= COEF_TOL
+    sure_p = ref["gap_pos"] >= GPU_EXACT_GAP
+    sure_n = ref["gap_neg"] >= GPU_EXACT_GAP
+    assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
+    assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
+
+
 def test_edge_one_and_two_events(oracle):
     """N=1 (diagonal only) and N=2 through the drop-in."""
     ev, _ = synth.make_events(2, 1, 1024, seed=71)

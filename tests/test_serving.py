@@ -89,6 +89,50 @@ def test_serving_matches_f64():
 
 
 @pytest.mark.gpu
+def test_serving_periods_and_wrapped_lags():
+    """Query lengths on both sides of every period boundary of a 1200-sample
+    catalog (P = 3072, 4096, 6144, 8192: P >= Lq + Lc - 1 and P/2 >= Lc), and
+    exact shifted catalog copies at lags beyond P/2 (reported by the kernel
+    as lag - P and moved back) and at negative lags."""
+    cat, _ = synth.make_events(40, 2, 1200, seed=71)
+    srv = serving.Server(cat, max_query_len=6000, top_k=4, max_batch_events=16, n_slots=2)
+    lens = np.array([1, 1000, 1873, 1874, 2897, 2898, 4945, 4946, 6000, 4900, 5500, 1800],
+                    np.int32)
+    q, _ = synth.make_events(len(lens), 2, 6000, seed=72)
+    for e in range(len(lens)):
+        q[e, :, lens[e]:] = 0
+    copies = {9: (5, 3700), 10: (17, 4200), 11: (23, -300)}   # query -> (catalog event, lag)
+    for e, (c, lag) in copies.items():
+        q[e] = 0
+        if lag >= 0:
+            q[e, :, lag:lag + 1200] = cat[c]
+        else:
+            q[e, :, :1200 + lag] = cat[c][:, -lag:]
+    rid = srv.submit(q, lens)
+    v, l, idx, lat = srv.wait(rid)
+    ref = oracle_topk(q, lens, cat, 4)
+    worst = 0.0
+    for e in range(len(lens)):
+        rv = np.array([t[0] for t in ref[e]])
+        worst = max(worst, np.abs(v[e] - rv).max())
+        for r in range(4):
+            sep = (r == 0 or rv[r - 1] - rv[r] > 1e-5) and (r == 3 or rv[r] - rv[r + 1] > 1e-5)
+            if sep:
+                assert idx[e, r] == ref[e][r][2], (e, r)
+                if ref[e][r][3] > 1e-5:
+                    assert l[e, r] == ref[e][r][1], (e, r, l[e, r], ref[e][r][1])
+    print(f"serving periods: max |dcoef| {worst:.3e}, latency {lat:.1f} ms")
+    assert worst <= 1e-5
+    for e, (c, lag) in copies.items():
+        assert idx[e, 0] == c and l[e, 0] == lag, (e, idx[e, 0], l[e, 0])
+        # a partial copy (negative lag) overlaps samples -lag.. of the event only
+        x = cat[c].astype(np.float64)
+        want = 1.0 if lag >= 0 else np.mean(np.linalg.norm(x[:, -lag:], axis=1) / np.linalg.norm(x, axis=1))
+        assert abs(v[e, 0] - want) < 1e-5, (e, v[e, 0], want)
+    srv.close()
+
+
+@pytest.mark.gpu
 def test_result_same_alone_and_in_mix():
     cat, _ = synth.make_events(300, 3, 1024, seed=31)
     srv = serving.Server(cat, max_query_len=2000, top_k=10, max_batch_events=64, n_slots=3)
