@@ -201,6 +201,14 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
 // half-length sequence Z[k] = (Y[k] + conj Y[M-k]) + i e^{2 pi i k/P}
 // (Y[k] - conj Y[M-k]) and runs the inverse wave FFT.  On return lane l,
 // register r holds c[2q] (x) and c[2q+1] (y), q = wave_out_index(l, r).
+// Largest divisor of R that is at most cap (R = 48, cap 32: 24), so the load
+// groups tile the R bins of a lane exactly.
+constexpr int load_group(int R, int cap) {
+  int g = R < cap ? R : cap;
+  while (R % g) --g;
+  return g;
+}
+
 template <int R, bool XI_GLOBAL = false, bool CT = false, int kLoadGroup = 32>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
@@ -215,7 +223,8 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     const float2* bj = xj + c * xj_stride;
     // All of a group's global loads are issued before the compiler fence so
     // they are in flight together (one L2 round trip per group, not per row).
-    constexpr int kGroup = R < kLoadGroup ? R : kLoadGroup;
+    constexpr int kGroup = load_group(R, kLoadGroup);
+    static_assert(R % kGroup == 0, "load groups must tile the lane's bins");
 #pragma unroll
     for (int m0 = 0; m0 < R; m0 += kGroup) {
       float2 Bv[kGroup];
