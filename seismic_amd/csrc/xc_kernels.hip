@@ -420,13 +420,13 @@ __device__ __forceinline__ long long packed_index(int i, int j, int n) {
 }
 
 // ----------------------------------------------------------------------------
-// pair kernel: one workgroup = 4 waves per work item (row i, column chunk)
+// pair kernel: one workgroup (kWaves waves) per work item (row i, column chunk)
 // ----------------------------------------------------------------------------
 // XI_LDS: the row spectra of a work item are staged in LDS (shared by the
 // workgroup's waves); otherwise (when they do not fit next to the tables,
-// e.g. P = 8192 with 3 channels) every wave reads them from global memory.
-// CT: compact twiddle tables (they leave room for the row spectra of larger
-// P x channel counts at the price of one extra complex multiply per twiddle).
+// e.g. P = 8192 with 5 channels) every wave reads them from global memory.
+// CT: compact twiddle tables (9 KB instead of 66 KB at P = 8192, one extra
+// complex multiply per twiddle; fewer LDS reads per pair).
 template <int R, bool FULL, int kWaves, bool XI_LDS, bool CT>
 __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArgs a) {
   constexpr int M = 64 * R;
@@ -467,9 +467,9 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
     for (int j = (dense ? jb : max(jb, i)) + wv; j < je; j += kWaves) {
       cf y[R];
       const float2* xj = cols + (long long)j * a.n_channels * a.spec_stride;
-      // rows from global memory double the loads in flight: smaller groups at
-      // P >= 6144 keep them within the 256-VGPR budget
-      pair_inverse<R, !XI_LDS, CT, (!XI_LDS && R >= 48) ? 16 : 32>(
+      // smaller load groups at P >= 6144 where registers are short: rows from
+      // global memory (twice the loads in flight) and the 12-wave budget
+      pair_inverse<R, !XI_LDS, CT, (R >= 48 && (!XI_LDS || kWaves == 12)) ? 16 : 32>(
           y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
@@ -618,20 +618,18 @@ static hipError_t launch_pairs_rfw(const PairArgs& a, int grid, hipStream_t st) 
 
 template <int R, bool FULL>
 static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
+  // Compact twiddle tables throughout (fewer LDS reads per pair for a few
+  // more multiplies: measured 1-1.3% faster at P <= 4096, 22% at P = 6144).
   // 12 waves (3 per SIMD) when the staged row spectra fit the 160 KiB of a
-  // CU, else 8; then 8 waves with the compact tables (P >= 4096 only); rows
-  // from global memory when none of these fit.  P = 6144 and 8192 always run
-  // 8 waves: their 96 / 128 spectrum registers per lane need the 256-VGPR
+  // CU, else 8; rows from global memory when neither fits.  P = 8192 always
+  // runs 8 waves: its 128 spectrum registers per lane need the 256-VGPR
   // budget.
   constexpr size_t kLds = 160 * 1024;
   const int C = a.n_channels;
-  if constexpr (R <= 32) {
-    if (pair_lds_bytes(R, C, 12, false) <= kLds) return launch_pairs_rfw<R, FULL, 12, true, false>(a, grid, st);
+  if constexpr (R <= 48) {
+    if (pair_lds_bytes(R, C, 12, true) <= kLds) return launch_pairs_rfw<R, FULL, 12, true, true>(a, grid, st);
   }
-  if (pair_lds_bytes(R, C, 8, false) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, false>(a, grid, st);
-  if constexpr (R >= 32) {
-    if (pair_lds_bytes(R, C, 8, true) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, true>(a, grid, st);
-  }
+  if (pair_lds_bytes(R, C, 8, true) <= kLds) return launch_pairs_rfw<R, FULL, 8, true, true>(a, grid, st);
   return launch_pairs_rfw<R, FULL, 8, false, false>(a, grid, st);
 }
 
