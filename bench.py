@@ -43,6 +43,49 @@ def fft_flops_per_pair(C: int, P: int) -> float:
     return C * 8.0 * B + 2.5 * P * math.log2(P)
 
 
+def fp64_reference(ev, P, n_pairs, seed, windows):
+    """FP64 numpy reference on seeded packed pairs (i <= j): the channel-
+    averaged normalized period-P circular correlation (the FFT form's
+    definition; equal to the linear one on every lag with |l| <= P - L), its
+    maximum above 0 and minimum below 0 with the first (smallest) lag, over
+    each window (lo, hi).  Returns the packed indices and per window
+    {pos, lag_pos, neg, lag_neg, gap_pos, gap_neg}."""
+    N, C, L = ev.shape
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = rng.integers(0, N, size=n_pairs)
+    b = rng.integers(0, N, size=n_pairs)
+    pi, pj = np.minimum(a, b), np.maximum(a, b)
+    idx = pi.astype(np.int64) * N - pi.astype(np.int64) * (pi - 1) // 2 + (pj - pi)
+    x = ev.astype(np.float64)
+    norms = np.linalg.norm(x, axis=2)                      # [N, C]
+    used = np.unique(np.concatenate([pi, pj]))
+    slot = np.full(N, -1, np.int64)
+    slot[used] = np.arange(used.size)
+    X = np.fft.rfft(x[used], P, axis=2) / norms[used][:, :, None]
+    r = np.fft.irfft((X[slot[pi]] * np.conj(X[slot[pj]])).sum(axis=1), P, axis=1) / C
+    out = {}
+    for lo, hi in windows:
+        lags = np.arange(lo, hi + 1)
+        cur = r[:, lags % P]
+        srt = np.sort(cur, axis=1)
+        kp, kn = cur.argmax(axis=1), cur.argmin(axis=1)
+        vp, vn = cur[np.arange(n_pairs), kp], cur[np.arange(n_pairs), kn]
+        out[(lo, hi)] = {"pos": np.where(vp > 0, vp, 0.0), "lag_pos": np.where(vp > 0, lags[kp], 0),
+                         "neg": np.where(vn < 0, vn, 0.0), "lag_neg": np.where(vn < 0, lags[kn], 0),
+                         "gap_pos": srt[:, -1] - srt[:, -2], "gap_neg": srt[:, 1] - srt[:, 0]}
+    return idx, out
+
+
+def vs_fp64(outs, idx, ref):
+    """Coefficient error and lag agreement of packed outputs (vp, lp, vn, ln)
+    against the FP64 reference on the sampled pairs."""
+    vp, lp, vn, ln = outs
+    return {"pairs": int(idx.size),
+            "max_abs_dcoef": float(max(np.abs(vp[idx] - ref["pos"]).max(), np.abs(vn[idx] - ref["neg"]).max())),
+            "lag_agreement": float(((lp[idx] == ref["lag_pos"]) & (ln[idx] == ref["lag_neg"])).mean()),
+            "fp64_near_ties_below_1e-5": int(((ref["gap_pos"] < 1e-5) | (ref["gap_neg"] < 1e-5)).sum())}
+
+
 def time_plan(plan, tr, outs, stream, steps, warmup):
     from seismic_amd.device import synchronize
     for _ in range(warmup):
@@ -59,8 +102,10 @@ def time_plan(plan, tr, outs, stream, steps, warmup):
     return ms, plan.timing(), plan.stats()
 
 
-def lag_forms(ev, tr, stream, steps, warmup):
-    """MFMA lag-window forms and the FFT form on the same +-LAG_WINDOW window."""
+def lag_forms(ev, tr, stream, steps, warmup, f64=None):
+    """MFMA lag-window forms and the FFT form on the same +-LAG_WINDOW window;
+    f64 = (packed indices, FP64 reference on that window) adds each form's
+    error against FP64."""
     from seismic_amd import engine
     N, C, L = ev.shape
     w = LAG_WINDOW
@@ -100,6 +145,9 @@ def lag_forms(ev, tr, stream, steps, warmup):
                               "ms_per_step": round(ms, 3), "_outs": [o.numpy() for o in outs]}
     plan.close()
     # agreement with the FFT form on the same window (both against each other)
+    if f64 is not None:
+        for name in ("bf16x3", "fp16", "fft_same_window"):
+            out[name]["vs_fp64"] = vs_fp64(out[name]["_outs"], *f64)
     ref = out["fft_same_window"].pop("_outs")
     for name in ("bf16x3", "fp16"):
         o = out[name].pop("_outs")
@@ -227,6 +275,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lag-forms", action="store_true", help="skip the MFMA lag-window forms")
     ap.add_argument("--lag-steps", type=int, default=2)
+    ap.add_argument("--fp64-pairs", type=int, default=2000,
+                    help="seeded pairs checked against the FP64 numpy reference, every form")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-tiled", action="store_true", help="skip the tiled N x N leg")
     ap.add_argument("--tiled-n", type=int, default=16384,
@@ -290,10 +340,19 @@ def main():
                 "flops_per_pair": fpp, "pairs_per_launch": int(packed),
                 "kernel_ms": round(tm["ms_pairs"], 4)}
 
+    # accuracy: every form against an FP64 numpy reference on seeded pairs
+    log(f"bench: FP64 reference on {args.fp64_pairs} sampled pairs")
+    w = LAG_WINDOW
+    idx64, ref64 = fp64_reference(ev, P, args.fp64_pairs, 64, [(lag_lo, lag_hi), (-w, w)])
+    accuracy = {"reference": "numpy FP64, channel-averaged normalized correlation, seeded packed pairs",
+                "fft_f32_full_lags": vs_fp64([o.numpy() for o in outs], idx64, ref64[(lag_lo, lag_hi)])}
+
     lag = None
     if not args.no_lag_forms:
         log(f"bench: lag-window forms at +-{LAG_WINDOW}")
-        lag = lag_forms(ev, tr, stream, args.lag_steps, 1)
+        lag = lag_forms(ev, tr, stream, args.lag_steps, 1, (idx64, ref64[(-w, w)]))
+        for name in ("bf16x3", "fp16", "fft_same_window"):
+            accuracy[f"{name}_w{w}"] = lag[name]["vs_fp64"]
         log(f"bench: bf16x3 {lag['bf16x3']['ms_per_step']} ms, fp16 {lag['fp16']['ms_per_step']} ms, "
             f"fft {lag['fft_same_window']['ms_per_step']} ms")
 
@@ -325,6 +384,7 @@ def main():
                    "packed_pairs_computed": int(packed)},
         "stages_ms": {k: round(v, 4) for k, v in tm.items() if k != "executes"},
         "near_ties_per_step": st["near_ties"],
+        "accuracy_vs_fp64": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mfma_bf16x3": lag,

@@ -82,12 +82,12 @@ def test_n8192_l4096_w512_forms_vs_f64_sampled(oracle):
     """Precision configuration at full size: N=8192, 1 channel, L=4096,
     window +-512, all 33.5M pairs in bf16x3, fp16 and the FP32-FFT form
     (P=8192, circular lags equal the linear ones since P >= L + 512);
-    the FP64 oracle on 10000 seeded pairs.  Prints the coefficient error and
+    the FP64 oracle on 100,000 seeded pairs.  Prints the coefficient error and
     the lag agreement rate of every form."""
     n, L, w = 8192, 4096, 512
     ev, _ = synth.make_events(n, 1, L, seed=8192)
     rng = np.random.Generator(np.random.PCG64(3))
-    k = 10000
+    k = 100000
     pi = rng.integers(0, n, size=k)
     pj = rng.integers(0, n, size=k)
     pi, pj = np.minimum(pi, pj).astype(np.int32), np.maximum(pi, pj).astype(np.int32)
