@@ -467,7 +467,7 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   ra.ties = p->ties;
   ra.tie_count = p->tie_count;
   ra.tie_cap = p->tie_cap;
-  HIP_CHECK(launch_resolve(p->R, ra, 2048, st));
+  HIP_CHECK(launch_resolve(p->R, ra, 8192, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
 }

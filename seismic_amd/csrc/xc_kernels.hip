@@ -518,16 +518,18 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   constexpr int P = 2 * M;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
-  float2* twl = smem;
-  float2* wl = smem + TwLayout<R>::kInv;
-  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kInv, lane, 64);
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
+  if (blockIdx.x >= total) return;  // no record for this workgroup
+  // compact tables (as the pair kernel): 11.8 KB of LDS at P = 4096
+  float2* twl = smem;
+  float2* wl = smem + TwLayoutCT<R>::kInv;
+  stage_twiddles<R>(twl, a.tw + TwLayout<R>::kInv, TwLayoutCT<R>::kInv, lane, 64);
   for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
     const int4 t = a.ties[rec];
     const int i = t.x, j = t.y, side = t.z;
     wave_lds_fence();
     cf y[R];
-    pair_inverse<R, true>(y, a.spec + (long long)i * a.n_channels * a.spec_stride, (int)a.spec_stride,
+    pair_inverse<R, true, true>(y, a.spec + (long long)i * a.n_channels * a.spec_stride, (int)a.spec_stride,
                           a.spec + (long long)j * a.n_channels * a.spec_stride, a.spec_stride,
                           a.n_channels, twl, wl, lane);
     apply_window<R>(y, a.winmask, lane);
@@ -545,19 +547,31 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) bv = fmaxf(bv, __shfl_xor(bv, o, 64));
       const float cut = bv - a.tie_tol;
-      double best = 0.0;
-      int best_lag = 0;
-      bool have = false;
+      // candidate slots: one ballot per slot, kept in this wave's LDS buffer
+      // (free after the transform), so the FP64 evaluation below exists once
+      // in the code instead of once per register slot
+      unsigned long long* cmask = reinterpret_cast<unsigned long long*>(wl);
       sfor<0, 2 * R>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const float v = sg * ((s & 1) ? y[s >> 1].y : y[s >> 1].x);
         const int lg = slot_lag<R>(lane, s);
         const bool inw = lg >= a.lag_lo && lg <= a.lag_hi;
-        unsigned long long mask = __ballot(inw && v >= cut && v > 0.f);
+        const unsigned long long m = __ballot(inw && v >= cut && v > 0.f);
+        if (lane == 0) cmask[s] = m;
+      });
+      wave_lds_fence();
+      double best = 0.0;
+      int best_lag = 0;
+      bool have = false;
+      for (int s = 0; s < 2 * R; ++s) {
+        const unsigned long long ms = cmask[s];
+        unsigned long long mask =
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(ms >> 32)) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)ms);
         while (mask) {
           const int src = __ffsll((long long)mask) - 1;
           mask &= mask - 1;
-          const int lag = __shfl(lg, src, 64);
+          const int lag = slot_lag<R>(src, s);
           double acc = 0.0;
           for (int c = 0; c < a.n_channels; ++c) {
             const double ni = a.norms[(long long)i * a.n_channels + c];
@@ -574,7 +588,8 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
             have = true;
           }
         }
-      });
+      }
+      wave_lds_fence();  // the next side or record rewrites the buffer
       if (lane == 0) {
         const float v = (float)best;
         if (sd == 0) {
@@ -640,7 +655,7 @@ hipError_t launch_pairs_r(bool full, const PairArgs& a, int grid, hipStream_t st
 
 template <int R>
 hipError_t launch_resolve_r(const ResolveArgs& a, int grid, hipStream_t st) {
-  size_t lds = (size_t)(TwLayout<R>::kInv + kWaveLds) * sizeof(float2);
+  size_t lds = (size_t)(TwLayoutCT<R>::kInv + kWaveLds) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_resolve<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_resolve<R>, dim3(grid), dim3(64), lds, st, a);
