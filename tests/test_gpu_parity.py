@@ -224,6 +224,34 @@ def test_extended_mixed_radix_periods(oracle, P, C, full):
     assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
 
 
+def test_dropin_padded_3072(oracle):
+    """The raw correlationCUDA entry point with paddedSize = 3072 (a period a
+    cuFFT plan accepts; the reference's caller only passes powers of two):
+    period-3072 circular correlation, lags -shift+1 .. shift."""
+    import ctypes
+    n, L, P, shift = 24, 1500, 3072, 1500
+    ev, _ = synth.make_events(n, 1, L, seed=3072)
+    rows = np.zeros((n, P), np.float32)
+    rows[:, :L] = ev[:, 0]
+    m = n * (n + 1) // 2
+    vp, vn = np.zeros(m, np.float32), np.zeros(m, np.float32)
+    lp, ln = np.zeros(m, np.int32), np.zeros(m, np.int32)
+    lib = correlation_lib.initialise_library()
+    fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    print("case drop-in N=24 L=1500 paddedSize=3072", flush=True)
+    lib.correlationCUDA(fp(rows), fp(rows), n, L, shift, P, 1, 0.0, fp(vp), ip(lp), fp(vn), ip(ln))
+    assert lib.seismic_last_error() == b""
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, -shift + 1, shift, P=P, circular=True)
+    dp, dn = np.abs(vp - ref["pos"]).max(), np.abs(vn - ref["neg"]).max()
+    print(f"drop-in P=3072: max |dcoef| pos {dp:.3e} neg {dn:.3e}")
+    assert dp <= COEF_TOL and dn <= COEF_TOL
+    sure_p, sure_n = ref["gap_pos"] >= GPU_EXACT_GAP, ref["gap_neg"] >= GPU_EXACT_GAP
+    assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
+    assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
+
+
 def test_edge_one_and_two_events(oracle):
     """N=1 (diagonal only) and N=2 through the drop-in."""
     ev, _ = synth.make_events(2, 1, 1024, seed=71)
