@@ -52,6 +52,9 @@ def test_invalid_server_desc_rejected():
         serving.Server(cat, max_query_len=100, top_k=40)
     with pytest.raises(RuntimeError, match="memory budget"):
         serving.Server(cat, max_query_len=100, top_k=2, mem_budget=1000)
+    # P/2 >= catalog length: no supported period above 8192
+    with pytest.raises(RuntimeError, match="FFT period above 8192"):
+        serving.Server(np.zeros((4, 1, 5000), np.float32), max_query_len=100, top_k=2)
 
 
 @pytest.mark.gpu
@@ -129,6 +132,39 @@ def test_serving_periods_and_wrapped_lags():
         x = cat[c].astype(np.float64)
         want = 1.0 if lag >= 0 else np.mean(np.linalg.norm(x[:, -lag:], axis=1) / np.linalg.norm(x, axis=1))
         assert abs(v[e, 0] - want) < 1e-5, (e, v[e, 0], want)
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_serving_catalog_longer_than_queries():
+    """Catalog events of 3000 samples, queries of 1..1000: P/2 >= L_c alone
+    sets P = 6144, and negative lags down to -(L_c - 1) come out unwrapped
+    (exact partial copies of catalog events at lags -2500 and -10)."""
+    cat, _ = synth.make_events(30, 1, 3000, seed=81)
+    srv = serving.Server(cat, max_query_len=1000, top_k=3, max_batch_events=8, n_slots=1)
+    lens = np.array([1000, 1000, 700, 1, 400], np.int32)
+    q, _ = synth.make_events(len(lens), 1, 1000, seed=82)
+    for e in range(len(lens)):
+        q[e, :, lens[e]:] = 0
+    copies = {0: (4, -2500), 1: (11, -10)}      # query -> (catalog event, lag)
+    for e, (c, lag) in copies.items():
+        q[e] = 0
+        n = min(lens[e], 3000 + lag)
+        q[e, :, :n] = cat[c][:, -lag:-lag + n]
+    v, l, idx, lat = srv.wait(srv.submit(q, lens))
+    ref = oracle_topk(q, lens, cat, 3)
+    worst = 0.0
+    for e in range(len(lens)):
+        rv = np.array([t[0] for t in ref[e]])
+        worst = max(worst, np.abs(v[e] - rv).max())
+        if rv[0] - rv[1] > 1e-5:
+            assert idx[e, 0] == ref[e][0][2]
+            if ref[e][0][3] > 1e-5:
+                assert l[e, 0] == ref[e][0][1], (e, l[e, 0], ref[e][0][1])
+    print(f"serving long catalog: max |dcoef| {worst:.3e}")
+    assert worst <= 1e-5
+    for e, (c, lag) in copies.items():
+        assert idx[e, 0] == c and l[e, 0] == lag, (e, idx[e, 0], l[e, 0])
     srv.close()
 
 
