@@ -88,6 +88,13 @@ hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream
 // columns pad with (-inf, 0, -1).
 hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int k, float* out_v,
                        int* out_l, int* out_idx, hipStream_t st);
+// The same result in two passes for few rows: each row's columns in
+// n_slices slices (one workgroup each) into tmp [n_rows][n_slices * k]
+// (value, lag, column), then the top-k of those candidates per row.
+constexpr int kTopkMaxSlices = 16;
+hipError_t launch_topk_split(const float* v, const int* l, int n_rows, int n_cols, int k, int n_slices,
+                             float* tmp_v, int* tmp_l, int* tmp_idx, float* out_v, int* out_l, int* out_idx,
+                             hipStream_t st);
 
 // Histograms and threshold over packed results computed without a threshold
 // (the OMP ABI's histograms count every pair before its threshold).

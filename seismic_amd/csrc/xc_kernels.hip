@@ -682,47 +682,58 @@ hipError_t launch_copy_f4(float4* dst, const float4* src, long long n, hipStream
   return hipGetLastError();
 }
 
-// One workgroup (256 threads) per row: each thread keeps a sorted list of its
-// K best (K >= k, a compile-time capacity) over a strided share of the
-// columns in registers -- a candidate not better than the thread's K-th best
-// is rejected with one comparison -- then wave 0 merges the 256 lists with k
-// rounds of a wave-wide argmax.  Ties go to the smaller column.
+// One workgroup (256 threads) per (row, column slice): each thread keeps a
+// sorted list of its K best (K >= k, a compile-time capacity) over a strided
+// share of the slice in registers -- a candidate not better than the thread's
+// K-th best is rejected with one comparison -- then wave 0 merges the 256
+// lists with k rounds of a wave-wide argmax.  Order: larger value, then
+// smaller index (idx_in[c] when given, else the column c).  The k winners
+// (value, lag, index) of slice s go to out[(row * slices + s) * k ...];
+// fewer than k candidates pad with (-inf, 0, -1).
 constexpr int kTopkMax = 32;
 __device__ __forceinline__ bool better(float v, int c, float bv, int bc) {
   return v > bv || (v == bv && c < bc);
 }
 template <int K>
 __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const int* __restrict__ l,
+                                              const int* __restrict__ idx_in, long long row_stride,
                                               int n_cols, int k, float* out_v, int* out_l, int* out_idx) {
   __shared__ float cv[256 * K];
   __shared__ int cc[256 * K];
-  const int row = blockIdx.x;
-  const float* rv = v + (long long)row * n_cols;
+  __shared__ int cp[256 * K];
+  const int row = blockIdx.x, slice = blockIdx.y, n_slices = gridDim.y;
+  const int width = (n_cols + n_slices - 1) / n_slices;
+  const int cb = slice * width, ce = min(n_cols, cb + width);
+  const long long base = (long long)row * row_stride;
+  const float* rv = v + base;
   float tv[K];
-  int tc[K];
+  int tc[K], tp[K];
 #pragma unroll
-  for (int q = 0; q < K; ++q) { tv[q] = -INFINITY; tc[q] = 0x7fffffff; }
+  for (int q = 0; q < K; ++q) { tv[q] = -INFINITY; tc[q] = 0x7fffffff; tp[q] = 0; }
   // eight loads in flight per thread, then the eight candidates in order
   constexpr int kBatch = 8;
-  for (int c0 = threadIdx.x; c0 < n_cols; c0 += 256 * kBatch) {
+  for (int c0 = cb + threadIdx.x; c0 < ce; c0 += 256 * kBatch) {
     float xb[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       const int c = c0 + 256 * u;
-      xb[u] = c < n_cols ? rv[c] : -INFINITY;
+      xb[u] = c < ce ? rv[c] : -INFINITY;
     }
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
+      const int c = c0 + 256 * u;
+      if (c >= ce) continue;
       float x = xb[u];
-      int xc = c0 + 256 * u;
-      if (xc >= n_cols || !better(x, xc, tv[K - 1], tc[K - 1])) continue;
+      int xc = idx_in ? idx_in[base + c] : c;
+      int xp = c;
+      if (!better(x, xc, tv[K - 1], tc[K - 1])) continue;
 #pragma unroll
       for (int q = 0; q < K; ++q) {  // insertion, keeps tv sorted descending
         if (better(x, xc, tv[q], tc[q])) {
           const float sv = tv[q];
-          const int sc = tc[q];
-          tv[q] = x; tc[q] = xc;
-          x = sv; xc = sc;
+          const int sc = tc[q], sp = tp[q];
+          tv[q] = x; tc[q] = xc; tp[q] = xp;
+          x = sv; xc = sc; xp = sp;
         }
       }
     }
@@ -731,6 +742,7 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const
   for (int q = 0; q < K; ++q) {
     cv[threadIdx.x * K + q] = tv[q];
     cc[threadIdx.x * K + q] = tc[q];
+    cp[threadIdx.x * K + q] = tp[q];
   }
   __syncthreads();
   if (threadIdx.x >= 64) return;
@@ -738,46 +750,73 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ v, const
   int head[4] = {0, 0, 0, 0};  // next unused candidate of the 4 lists this lane owns
   for (int r = 0; r < k; ++r) {
     float bv = -INFINITY;
-    int bc = 0x7fffffff, bl = -1;
+    int bc = 0x7fffffff, bp = 0, bl = -1;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int t = lane + 64 * u;
       if (head[u] < K) {
         const float x = cv[t * K + head[u]];
         const int xc = cc[t * K + head[u]];
-        if (better(x, xc, bv, bc)) { bv = x; bc = xc; bl = u; }
+        if (better(x, xc, bv, bc)) { bv = x; bc = xc; bp = cp[t * K + head[u]]; bl = u; }
       }
     }
-    // wave argmax over (bv, bc)
+    // wave argmax over (bv, bc), carrying the winner's position
     float wv = bv;
-    int wc = bc;
+    int wc = bc, wp = bp;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       const float ov = __shfl_xor(wv, o, 64);
       const int oc = __shfl_xor(wc, o, 64);
-      if (better(ov, oc, wv, wc)) { wv = ov; wc = oc; }
+      const int op = __shfl_xor(wp, o, 64);
+      if (better(ov, oc, wv, wc)) { wv = ov; wc = oc; wp = op; }
     }
     if (bl >= 0 && bc == wc && bv == wv) head[bl]++;  // the owner advances its list
     if (lane == 0) {
-      const long long o = (long long)row * k + r;
+      const long long o = ((long long)row * n_slices + slice) * k + r;
       const bool ok = wc != 0x7fffffff;
       out_v[o] = ok ? wv : -INFINITY;
       out_idx[o] = ok ? wc : -1;
-      out_l[o] = ok ? l[(long long)row * n_cols + wc] : 0;
+      out_l[o] = ok ? l[base + wp] : 0;
     }
   }
+}
+
+template <int K>
+static void launch_topk_k(const float* v, const int* l, const int* idx_in, long long row_stride, int n_rows,
+                          int n_slices, int n_cols, int k, float* out_v, int* out_l, int* out_idx,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(k_topk<K>, dim3(n_rows, n_slices), dim3(256), 0, st, v, l, idx_in, row_stride, n_cols, k,
+                     out_v, out_l, out_idx);
+}
+
+static void launch_topk_any(const float* v, const int* l, const int* idx_in, long long row_stride, int n_rows,
+                            int n_slices, int n_cols, int k, float* out_v, int* out_l, int* out_idx,
+                            hipStream_t st) {
+  if (k <= 8)
+    launch_topk_k<8>(v, l, idx_in, row_stride, n_rows, n_slices, n_cols, k, out_v, out_l, out_idx, st);
+  else if (k <= 16)
+    launch_topk_k<16>(v, l, idx_in, row_stride, n_rows, n_slices, n_cols, k, out_v, out_l, out_idx, st);
+  else
+    launch_topk_k<32>(v, l, idx_in, row_stride, n_rows, n_slices, n_cols, k, out_v, out_l, out_idx, st);
 }
 
 hipError_t launch_topk(const float* v, const int* l, int n_rows, int n_cols, int k, float* out_v,
                        int* out_l, int* out_idx, hipStream_t st) {
   if (n_rows <= 0) return hipSuccess;
   if (k < 1 || k > kTopkMax) return hipErrorInvalidValue;
-  if (k <= 8)
-    hipLaunchKernelGGL(k_topk<8>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
-  else if (k <= 16)
-    hipLaunchKernelGGL(k_topk<16>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
-  else
-    hipLaunchKernelGGL(k_topk<32>, dim3(n_rows), dim3(256), 0, st, v, l, n_cols, k, out_v, out_l, out_idx);
+  launch_topk_any(v, l, nullptr, n_cols, n_rows, 1, n_cols, k, out_v, out_l, out_idx, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_topk_split(const float* v, const int* l, int n_rows, int n_cols, int k, int n_slices,
+                             float* tmp_v, int* tmp_l, int* tmp_idx, float* out_v, int* out_l, int* out_idx,
+                             hipStream_t st) {
+  if (n_rows <= 0) return hipSuccess;
+  if (k < 1 || k > kTopkMax || n_slices < 1 || n_slices > kTopkMaxSlices) return hipErrorInvalidValue;
+  if (n_slices == 1) return launch_topk(v, l, n_rows, n_cols, k, out_v, out_l, out_idx, st);
+  launch_topk_any(v, l, nullptr, n_cols, n_rows, n_slices, n_cols, k, tmp_v, tmp_l, tmp_idx, st);
+  const int m = n_slices * k;  // candidates per row
+  launch_topk_any(tmp_v, tmp_l, tmp_idx, m, n_rows, 1, m, k, out_v, out_l, out_idx, st);
   return hipGetLastError();
 }
 

@@ -118,6 +118,9 @@ struct Slot {
   float* d_tv = nullptr;       // [B][k]
   int* d_tl = nullptr;
   int* d_ti = nullptr;
+  float* d_kv = nullptr;       // [B][kTopkMaxSlices * k] top-k candidates per column slice
+  int* d_kl = nullptr;
+  int* d_ki = nullptr;
   int2* d_work = nullptr;      // [B * n_chunks]
   unsigned* d_tie = nullptr;
   float* h_traces = nullptr;   // pinned staging
@@ -161,7 +164,7 @@ namespace {
 
 void slot_free(Slot* s) {
   void* dev[] = {s->d_traces, s->d_len, s->d_spec, s->d_norms, s->d_sv, s->d_sl,
-                 s->d_tv, s->d_tl, s->d_ti, s->d_work, s->d_tie};
+                 s->d_tv, s->d_tl, s->d_ti, s->d_kv, s->d_kl, s->d_ki, s->d_work, s->d_tie};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   void* host[] = {s->h_traces, s->h_len, s->h_work, s->h_tv, s->h_tl, s->h_ti};
@@ -175,6 +178,7 @@ unsigned long long slot_bytes(const sxc_server_desc& d, long long spec_stride_ma
   const unsigned long long B = d.max_batch_events, C = d.n_channels;
   return B * C * (unsigned long long)d.max_query_len * 4 + B * 4 + B * C * spec_stride_max * 8 +
          B * C * 8 + B * (unsigned long long)d.n_catalog * 8 + B * d.top_k * 12 +
+         B * kTopkMaxSlices * d.top_k * 12 +
          B * (unsigned long long)n_chunks * 8 + 64;
 }
 
@@ -191,6 +195,9 @@ int slot_create(sxc_server* S, Slot* s) {
   SRV_CHECK(hipMalloc((void**)&s->d_tv, B * d.top_k * sizeof(float)));
   SRV_CHECK(hipMalloc((void**)&s->d_tl, B * d.top_k * sizeof(int)));
   SRV_CHECK(hipMalloc((void**)&s->d_ti, B * d.top_k * sizeof(int)));
+  SRV_CHECK(hipMalloc((void**)&s->d_kv, B * kTopkMaxSlices * d.top_k * sizeof(float)));
+  SRV_CHECK(hipMalloc((void**)&s->d_kl, B * kTopkMaxSlices * d.top_k * sizeof(int)));
+  SRV_CHECK(hipMalloc((void**)&s->d_ki, B * kTopkMaxSlices * d.top_k * sizeof(int)));
   SRV_CHECK(hipMalloc((void**)&s->d_work, B * S->n_chunks * sizeof(int2)));
   SRV_CHECK(hipMalloc((void**)&s->d_tie, sizeof(unsigned)));
   SRV_CHECK(hipHostMalloc((void**)&s->h_traces, B * C * d.max_query_len * sizeof(float), hipHostMallocDefault));
@@ -266,8 +273,13 @@ int run_batch(sxc_server* S, Slot* s, const std::vector<std::pair<int, int>>& gr
     pa.tie_cap = 0;  // serving ranks by FP32 values; no FP64 tie records
     const int grid = pa.n_work >= 4096 ? 4096 : (pa.n_work >= 8 ? pa.n_work / 8 * 8 : pa.n_work);
     SRV_CHECK(launch_pairs(T.R, true, pa, grid, s->stream));
-    SRV_CHECK(launch_topk(pa.vpos, pa.lpos, cnt, d.n_catalog, d.top_k, s->d_tv + (size_t)first * d.top_k,
-                          s->d_tl + (size_t)first * d.top_k, s->d_ti + (size_t)first * d.top_k, s->stream));
+    // top-k: a few rows would leave most CUs idle, so each row's columns are
+    // split over slices (about 512 workgroups, at least 2048 columns each)
+    const int slices = std::max(1, std::min({kTopkMaxSlices, (512 + cnt - 1) / cnt, d.n_catalog / 2048}));
+    const size_t kt = (size_t)first * kTopkMaxSlices * d.top_k;
+    SRV_CHECK(launch_topk_split(pa.vpos, pa.lpos, cnt, d.n_catalog, d.top_k, slices, s->d_kv + kt, s->d_kl + kt,
+                                s->d_ki + kt, s->d_tv + (size_t)first * d.top_k,
+                                s->d_tl + (size_t)first * d.top_k, s->d_ti + (size_t)first * d.top_k, s->stream));
   }
   SRV_CHECK(hipMemcpyAsync(s->h_tv, s->d_tv, (size_t)n_total * d.top_k * sizeof(float), hipMemcpyDeviceToHost, s->stream));
   SRV_CHECK(hipMemcpyAsync(s->h_tl, s->d_tl, (size_t)n_total * d.top_k * sizeof(int), hipMemcpyDeviceToHost, s->stream));
