@@ -284,8 +284,8 @@ def main():
     ap.add_argument("--tiled-block", type=int, default=8192)
     ap.add_argument("--serve-catalog", type=int, default=32768)
     ap.add_argument("--serve-stream-requests", type=int, default=64)
-    ap.add_argument("--serve-64-requests", type=int, default=100)
-    ap.add_argument("--serve-64-rates", type=str, default="5,10,15,20,30",
+    ap.add_argument("--serve-64-requests", type=int, default=200)
+    ap.add_argument("--serve-64-rates", type=str, default="5,10,20,30",
                     help="comma-separated Poisson rates (req/s) of the 64-event requests")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (scales PMC byte counters)")
