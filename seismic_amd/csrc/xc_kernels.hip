@@ -519,12 +519,19 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
-  if (blockIdx.x >= total) return;  // no record for this workgroup
+  // XCD-aware walk: XCD x = blockIdx % 8 resolves its own contiguous eighth
+  // of the record list (records arrive roughly grouped by tile or row), so
+  // the traces a record re-reads are usually still in that XCD's L2
+  const unsigned n_xcd = (gridDim.x % 8 == 0) ? 8 : 1;
+  const unsigned xcd = blockIdx.x % n_xcd, local = blockIdx.x / n_xcd, n_local = gridDim.x / n_xcd;
+  const unsigned seg = (total + n_xcd - 1) / n_xcd;
+  const unsigned rec0 = xcd * seg + local, seg_end = min(total, (xcd + 1) * seg);
+  if (rec0 >= seg_end) return;  // no record for this workgroup
   // compact tables (as the pair kernel): 11.8 KB of LDS at P = 4096
   float2* twl = smem;
   float2* wl = smem + TwLayoutCT<R>::kInv;
   stage_twiddles<R>(twl, a.tw + TwLayout<R>::kInv, TwLayoutCT<R>::kInv, lane, 64);
-  for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
+  for (unsigned rec = rec0; rec < seg_end; rec += n_local) {
     const int4 t = a.ties[rec];
     const int i = t.x, j = t.y, side = t.z;
     wave_lds_fence();

@@ -363,7 +363,14 @@ __device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, con
 __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
   const int lane = threadIdx.x;
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
-  for (unsigned rec = blockIdx.x; rec < total; rec += gridDim.x) {
+  // XCD-aware walk: XCD x = blockIdx % 8 resolves its own contiguous eighth
+  // of the record list (records arrive roughly grouped by tile or row), so
+  // the traces a record re-reads are usually still in that XCD's L2
+  const unsigned n_xcd = (gridDim.x % 8 == 0) ? 8 : 1;
+  const unsigned xcd = blockIdx.x % n_xcd, local = blockIdx.x / n_xcd, n_local = gridDim.x / n_xcd;
+  const unsigned seg = (total + n_xcd - 1) / n_xcd;
+  const unsigned rec0 = xcd * seg + local, seg_end = min(total, (xcd + 1) * seg);
+  for (unsigned rec = rec0; rec < seg_end; rec += n_local) {
     const int4 t0 = a.ties[2 * rec], t1 = a.ties[2 * rec + 1];
     const int i = t0.x, j = t0.y, sides = t0.z;
     const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
