@@ -17,10 +17,12 @@
 // admission); a request larger than a slot is rejected at submit.
 //
 // Lags.  Query q (length Lq) against catalog event c (length Lc) uses the FFT
-// period P: the smallest power of two with P >= Lq + Lc - 1 and
-// P/2 >= max(Lq - 1, Lc), so the period-P correlation equals the linear one
-// on every lag that overlaps (-(Lc-1) .. Lq-1); lag > 0 means the query is
-// delayed relative to the catalog event.  Events of one batch are grouped by
+// period P: the smallest of 1024, 2048, 3072, 4096, 6144, 8192 with
+// P >= Lq + Lc - 1 and P/2 >= Lc, so the period-P correlation equals the
+// linear one on every lag that overlaps (-(Lc-1) .. Lq-1); lag > 0 means the
+// query is delayed relative to the catalog event.  Positive lags above P/2,
+// which the pair kernel reports as lag - P, are moved back when the results
+// are assembled (period_for, wrap_lag).  Events of one batch are grouped by
 // P and each group runs its own forward / pair / top-k launches.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
