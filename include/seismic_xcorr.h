@@ -55,7 +55,7 @@ typedef struct {
   int lag_hi;        /* last scanned lag >= lag_lo: FFT form <= P/2, lag forms < L */
   float threshold;   /* pos kept if > threshold, neg if < -threshold */
   float tie_tol;     /* near-tie band in coefficient units; <= 0 selects the form's default
-                        (FFT 4e-6, bf16x3 2e-5, fp16 2.5e-4) */
+                        (FFT 2e-6, bf16x3 2e-5, fp16 2.5e-4) */
   int demean;        /* 1: subtract the FP64 mean of the first L samples of each trace */
   int form;          /* SXC_FORM_* */
 } sxc_plan_desc;

@@ -337,9 +337,10 @@ int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   p->full = (desc->lag_lo == -p->M + 1 && desc->lag_hi == p->M);
   p->spec_stride = p->M + 32;  // 256-byte aligned rows, bins 0..M
   p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
-  // default band: 8x the largest FP32-vs-FP64 coefficient error measured
-  // (5e-7), so a pair outside the band has the FP64 winner
-  p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol : 4e-6f;
+  // default band: 4x the largest FP32-vs-FP64 coefficient error measured
+  // (5e-7); any band of at least twice the error leaves every pair outside it
+  // with the FP64 winner (a flipped winner has an FP32 gap below 2x the error)
+  p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol : 2e-6f;
   p->tie_cap = (int)std::min<long long>(p->n_pairs, 1LL << 24);
   HIP_CHECK(hipGetDevice(&p->device));
   const long long rows = (long long)desc->n_events * desc->n_channels;
