@@ -25,7 +25,7 @@ void initialiseCUDA(void);
  *                    reversed spectrum from the forward one)
  *   shift:           scanned lags are -shift+1 .. shift; needs
  *                    event_length - shift >= 0 and event_length + shift <= paddedSize
- *   paddedSize:      FFT period: 1024, 2048, 3072, 4096, 6144 or 8192 (the
+ *   paddedSize:      FFT period: 1024, 2048, 3072, 4096, 5120, 6144 or 8192 (the
  *                    reference's caller passes 1 << L.bit_length())
  *   num_threads:     unused (the reference only prints it)
  *   outputs:         packed upper triangle incl. diagonal, n_events*(n_events+1)/2

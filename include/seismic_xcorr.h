@@ -50,7 +50,7 @@ typedef struct {
   int n_events;      /* N >= 1 */
   int n_channels;    /* C in [1, 8]; channel correlations are averaged per lag */
   int trace_len;     /* L: samples entering the energy (and the mean) */
-  int padded_size;   /* FFT form: period P in {1024, 2048, 3072, 4096, 6144, 8192}, lags circular mod P */
+  int padded_size;   /* FFT form: period P in {1024, 2048, 3072, 4096, 5120, 6144, 8192}, lags circular mod P */
   int lag_lo;        /* first scanned lag: FFT form > -P/2, lag forms > -L */
   int lag_hi;        /* last scanned lag >= lag_lo: FFT form <= P/2, lag forms < L */
   float threshold;   /* pos kept if > threshold, neg if < -threshold */

@@ -7,8 +7,8 @@ Outputs
   lib/correlation_cuda.so      drop-in for Correlation_in_CUDA (include/correlation_cuda.h)
   lib/correlation_multicuda.so drop-in for Correlation_in_MultiCUDA (include/correlation_multicuda.h)
 The kernels are compiled for gfx950 only, one object per FFT size so the
-six instantiations (P = 1024 ... 8192, including 3072 and 6144) build in
-parallel.
+seven instantiations (P = 1024 ... 8192, including 3072, 5120 and 6144)
+build in parallel.
 """
 from __future__ import annotations
 
@@ -35,7 +35,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Wno-unused-result"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
               f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
-KERNEL_RADICES = (8, 16, 24, 32, 48, 64)
+KERNEL_RADICES = (8, 16, 24, 32, 40, 48, 64)
 HEADERS = ("xc_fft.hpp", "xc_kernels.h", "xc_engine.h", "xc_lag.h", "xc_plan.h")
 
 

@@ -12,6 +12,7 @@ hipError_t launch_forward(int R, const ForwardArgs& a, hipStream_t st) {
     case 16: return launch_forward_r<16>(a, st);
     case 24: return launch_forward_r<24>(a, st);
     case 32: return launch_forward_r<32>(a, st);
+    case 40: return launch_forward_r<40>(a, st);
     case 48: return launch_forward_r<48>(a, st);
     case 64: return launch_forward_r<64>(a, st);
   }
@@ -25,6 +26,7 @@ hipError_t launch_pairs(int R, bool full, const PairArgs& a, int grid, hipStream
     case 16: return launch_pairs_r<16>(full, a, grid, st);
     case 24: return launch_pairs_r<24>(full, a, grid, st);
     case 32: return launch_pairs_r<32>(full, a, grid, st);
+    case 40: return launch_pairs_r<40>(full, a, grid, st);
     case 48: return launch_pairs_r<48>(full, a, grid, st);
     case 64: return launch_pairs_r<64>(full, a, grid, st);
   }
@@ -37,6 +39,7 @@ hipError_t launch_resolve(int R, const ResolveArgs& a, int grid, hipStream_t st)
     case 16: return launch_resolve_r<16>(a, grid, st);
     case 24: return launch_resolve_r<24>(a, grid, st);
     case 32: return launch_resolve_r<32>(a, grid, st);
+    case 40: return launch_resolve_r<40>(a, grid, st);
     case 48: return launch_resolve_r<48>(a, grid, st);
     case 64: return launch_resolve_r<64>(a, grid, st);
   }

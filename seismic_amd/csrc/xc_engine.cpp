@@ -80,7 +80,7 @@ int validate_desc(const sxc_plan_desc& d) {
     return SXC_OK;
   }
   if (!fft_period_supported(P)) {
-    set_error("padded_size must be one of 1024, 2048, 3072, 4096, 6144, 8192 (got %d)", P);
+    set_error("padded_size must be one of 1024, 2048, 3072, 4096, 5120, 6144, 8192 (got %d)", P);
     return SXC_ERR_INVALID;
   }
   if (d.trace_len < 1 || d.trace_len > P) { set_error("trace_len must be in [1, padded_size] (got %d)", d.trace_len); return SXC_ERR_INVALID; }

@@ -13,9 +13,9 @@ void twiddle(long k, long n, int sign, float* re, float* im);
 std::vector<float2> wave_tables(int R, int sign);
 // lag held by (lane, slot) after the inverse wave FFT
 int slot_lag_host(int R, int lane, int slot);
-// FFT periods the kernels are built for: P = 128 R, R in {8, 16, 24, 32, 48, 64}
+// FFT periods the kernels are built for: P = 128 R, R in {8, 16, 24, 32, 40, 48, 64}
 inline bool fft_period_supported(int P) {
-  return P == 1024 || P == 2048 || P == 3072 || P == 4096 || P == 6144 || P == 8192;
+  return P == 1024 || P == 2048 || P == 3072 || P == 4096 || P == 5120 || P == 6144 || P == 8192;
 }
 void set_error(const char* fmt, ...);
 void clear_error();

@@ -181,6 +181,30 @@ __device__ __forceinline__ void bfly3(cf b0, cf b1, cf b2, cf& x0, cf& x1, cf& x
   x2 = cmk(__builtin_fmaf(s3, t2.y, m.x), __builtin_fmaf(-s3, t2.x, m.y));
 }
 
+// Radix-5 butterfly: X_q = sum_n b_n W5^(n q), W5 = exp(SIGN 2 pi i / 5).
+// With a1 = b1 + b4, d1 = b1 - b4, a2 = b2 + b3, d2 = b2 - b3 and
+// c1, c2 = cos(2 pi/5), cos(4 pi/5), s1, s2 = sin(2 pi/5), sin(4 pi/5):
+// X_1,4 = b0 + c1 a1 + c2 a2 +- i SIGN (s1 d1 + s2 d2),
+// X_2,3 = b0 + c2 a1 + c1 a2 +- i SIGN (s2 d1 - s1 d2).
+template <int SIGN>
+__device__ __forceinline__ void bfly5(cf b0, cf b1, cf b2, cf b3, cf b4, cf& x0, cf& x1, cf& x2, cf& x3,
+                                      cf& x4) {
+  constexpr float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  constexpr float s1 = SIGN * 0.95105651629515357212f, s2 = SIGN * 0.58778525229247312917f;
+  const cf a1 = cadd(b1, b4), d1 = csub(b1, b4), a2 = cadd(b2, b3), d2 = csub(b2, b3);
+  x0 = cadd(b0, cadd(a1, a2));
+  const cf m1 = cmk(__builtin_fmaf(c2, a2.x, __builtin_fmaf(c1, a1.x, b0.x)),
+                    __builtin_fmaf(c2, a2.y, __builtin_fmaf(c1, a1.y, b0.y)));
+  const cf m2 = cmk(__builtin_fmaf(c1, a2.x, __builtin_fmaf(c2, a1.x, b0.x)),
+                    __builtin_fmaf(c1, a2.y, __builtin_fmaf(c2, a1.y, b0.y)));
+  const cf n1 = cmk(__builtin_fmaf(s2, d2.x, s1 * d1.x), __builtin_fmaf(s2, d2.y, s1 * d1.y));
+  const cf n2 = cmk(__builtin_fmaf(-s1, d2.x, s2 * d1.x), __builtin_fmaf(-s1, d2.y, s2 * d1.y));
+  x1 = cmk(m1.x - n1.y, m1.y + n1.x);
+  x4 = cmk(m1.x + n1.y, m1.y - n1.x);
+  x2 = cmk(m2.x - n2.y, m2.y + n2.x);
+  x3 = cmk(m2.x + n2.y, m2.y - n2.x);
+}
+
 // In-place length-N DFT, N = 2^k: radix-2 DIT, natural order in and out.
 template <int N, int SIGN>
 __device__ __forceinline__ void reg_fft_pow2(cf (&a)[N]) {
@@ -207,12 +231,31 @@ __device__ __forceinline__ void reg_fft_pow2(cf (&a)[N]) {
 }
 
 // In-place length-N DFT of a register array, natural order in and out.
-// N = 2^k: radix-2 DIT.  N = 3 * 2^k: decimation in time by 3 -- three
-// length-N/3 DFTs of x[3n + r], then X[k + (N/3) q] = sum_r W_N^(r k)
-// W_3^(r q) F_r[k], one twiddled radix-3 butterfly per k.
+// N = 2^k: radix-2 DIT.  N = 3 * 2^k (5 * 2^k): decimation in time by 3 (5)
+// -- three (five) length-N/3 (N/5) DFTs of x[3n + r] (x[5n + r]), then
+// X[k + Q q] = sum_r W_N^(r k) W_3^(r q) F_r[k] (W_5), one twiddled radix-3
+// (radix-5) butterfly per k.
 template <int N, int SIGN>
 __device__ __forceinline__ void reg_fft(cf (&a)[N]) {
-  if constexpr (N % 3 == 0) {
+  if constexpr (N % 5 == 0) {
+    constexpr int Q = N / 5;
+    static_assert((Q & (Q - 1)) == 0, "N must be 2^k, 3 * 2^k or 5 * 2^k");
+    cf f[5][Q];
+    sfor<0, Q>([&](auto nc) {
+      constexpr int n = decltype(nc)::value;
+      sfor<0, 5>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        f[r][n] = a[5 * n + r];
+      });
+    });
+    sfor<0, 5>([&](auto rc) { reg_fft<Q, SIGN>(f[decltype(rc)::value]); });
+    sfor<0, Q>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      bfly5<SIGN>(f[0][k], twmul<k, N, SIGN>(f[1][k]), twmul<2 * k, N, SIGN>(f[2][k]),
+                  twmul<3 * k, N, SIGN>(f[3][k]), twmul<4 * k, N, SIGN>(f[4][k]), a[k], a[k + Q],
+                  a[k + 2 * Q], a[k + 3 * Q], a[k + 4 * Q]);
+    });
+  } else if constexpr (N % 3 == 0) {
     constexpr int Q = N / 3;
     static_assert((Q & (Q - 1)) == 0, "N must be 2^k or 3 * 2^k");
     cf f0[Q], f1[Q], f2[Q];

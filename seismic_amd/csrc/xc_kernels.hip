@@ -26,7 +26,7 @@
 #include "xc_kernels.h"
 
 #ifndef SXC_R
-#error "compile once per R with -DSXC_R=8|16|24|32|48|64"
+#error "compile once per R with -DSXC_R=8|16|24|32|40|48|64"
 #endif
 
 namespace sxc {

@@ -17,7 +17,7 @@
 // admission); a request larger than a slot is rejected at submit.
 //
 // Lags.  Query q (length Lq) against catalog event c (length Lc) uses the FFT
-// period P: the smallest of 1024, 2048, 3072, 4096, 6144, 8192 with
+// period P: the smallest of 1024, 2048, 3072, 4096, 5120, 6144, 8192 with
 // P >= Lq + Lc - 1 and P/2 >= Lc, so the period-P correlation equals the
 // linear one on every lag that overlaps (-(Lc-1) .. Lq-1); lag > 0 means the
 // query is delayed relative to the catalog event.  Positive lags above P/2,
@@ -56,7 +56,7 @@ using Clock = std::chrono::steady_clock;
     }                                                                                            \
   } while (0)
 
-constexpr int kPeriods[] = {1024, 2048, 3072, 4096, 6144, 8192};
+constexpr int kPeriods[] = {1024, 2048, 3072, 4096, 5120, 6144, 8192};
 
 // Smallest FFT period whose circular correlation equals the linear one on
 // every overlapping lag -(lc-1) .. lq-1 (P >= lq + lc - 1).  The pair kernel

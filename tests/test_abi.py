@@ -81,9 +81,9 @@ def test_invalid_plan_reported():
     from seismic_amd import engine
     with pytest.raises(RuntimeError, match="padded_size"):
         engine.AllPairsPlan(4, 1, 100, 1000, -10, 10)
-    # supported periods: P = 128 R, R in {8, 16, 24, 32, 48, 64}
-    for bad in (1536, 5120, 12288, 16384, 512):
-        with pytest.raises(RuntimeError, match="one of 1024, 2048, 3072, 4096, 6144, 8192"):
+    # supported periods: P = 128 R, R in {8, 16, 24, 32, 40, 48, 64}
+    for bad in (1536, 2560, 7168, 12288, 16384, 512):
+        with pytest.raises(RuntimeError, match="one of 1024, 2048, 3072, 4096, 5120, 6144, 8192"):
             engine.AllPairsPlan(4, 1, 100, bad, -10, 10)
     with pytest.raises(RuntimeError, match="lag window"):
         engine.AllPairsPlan(4, 1, 100, 1024, 10, -10)

@@ -200,13 +200,16 @@ def test_extended_p8192_channel_counts(oracle, C, full):
 
 
 @pytest.mark.parametrize("P,C,full", [(3072, 1, True), (3072, 3, False), (3072, 8, False),
+                                      (5120, 1, True), (5120, 3, False), (5120, 5, False),
+                                      (5120, 8, False),
                                       (6144, 1, True), (6144, 3, False), (6144, 4, False),
                                       (6144, 5, False)])
 def test_extended_mixed_radix_periods(oracle, P, C, full):
-    """Periods 3 * 2^k (a radix-3 stage in the register DFT), traces longer
-    than P/2 so the circular wrap matters.  P=3072: 12 waves (C <= 6), 8
-    waves (C=8); P=6144: 8 waves with full tables (C <= 3), compact tables
-    (C=4), rows from global memory (C=5)."""
+    """Periods 3 * 2^k and 5 * 2^k (a radix-3 or radix-5 stage in the
+    register DFT), traces longer than P/2 so the circular wrap matters, over
+    the kernel variants: 12 waves (P=3072 up to 8 channels, P=5120 and 6144
+    up to 3), 8 waves (P=5120 with 5, P=6144 with 4 channels), rows from
+    global memory (P=5120 with 8, P=6144 with 5 channels)."""
     from seismic_amd import engine
     n, L = 12, P - 200
     ev, _ = synth.make_events(n, C, L, seed=90 + C + P // 1024)

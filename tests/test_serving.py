@@ -94,17 +94,18 @@ def test_serving_matches_f64():
 @pytest.mark.gpu
 def test_serving_periods_and_wrapped_lags():
     """Query lengths on both sides of every period boundary of a 1200-sample
-    catalog (P = 3072, 4096, 6144, 8192: P >= Lq + Lc - 1 and P/2 >= Lc), and
-    exact shifted catalog copies at lags beyond P/2 (reported by the kernel
-    as lag - P and moved back) and at negative lags."""
+    catalog (P = 3072, 4096, 5120, 6144, 8192: P >= Lq + Lc - 1 and
+    P/2 >= Lc), and exact shifted catalog copies at lags beyond P/2 (reported
+    by the kernel as lag - P and moved back) and at negative lags."""
     cat, _ = synth.make_events(40, 2, 1200, seed=71)
     srv = serving.Server(cat, max_query_len=6000, top_k=4, max_batch_events=16, n_slots=2)
-    lens = np.array([1, 1000, 1873, 1874, 2897, 2898, 4945, 4946, 6000, 4900, 5500, 1800],
-                    np.int32)
+    lens = np.array([1, 1000, 1873, 1874, 2897, 2898, 4945, 4946, 6000, 4900, 5500, 1800,
+                     3921, 3922, 3900], np.int32)
     q, _ = synth.make_events(len(lens), 2, 6000, seed=72)
     for e in range(len(lens)):
         q[e, :, lens[e]:] = 0
-    copies = {9: (5, 3700), 10: (17, 4200), 11: (23, -300)}   # query -> (catalog event, lag)
+    # query -> (catalog event, lag): wrapped lags at P = 6144, 8192 and 5120
+    copies = {9: (5, 3700), 10: (17, 4200), 11: (23, -300), 14: (31, 2650)}
     for e, (c, lag) in copies.items():
         q[e] = 0
         if lag >= 0:
