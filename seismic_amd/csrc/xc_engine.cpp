@@ -90,8 +90,13 @@ int validate_desc(const sxc_plan_desc& d) {
   }
   if (d.form != SXC_FORM_FFT_F32) { set_error("unsupported form %d", d.form); return SXC_ERR_INVALID; }
   if (!(d.threshold == d.threshold)) { set_error("threshold is NaN"); return SXC_ERR_INVALID; }
-  long long pairs = (long long)d.n_events * (d.n_events + 1) / 2;
-  if (pairs > (1LL << 40)) { set_error("too many pairs"); return SXC_ERR_INVALID; }
+  // work items (row, 64-column chunk) are indexed with int: about N^2/128
+  long long items = 0;
+  for (long long j1 = 64; j1 - 64 < d.n_events && items <= 0x7fffffffLL; j1 += 64) items += std::min<long long>(j1, d.n_events);
+  if (items > 0x7fffffffLL) {
+    set_error("n_events %d exceeds the work-list limit of the FFT form (2^31 row x 64-column items)", d.n_events);
+    return SXC_ERR_INVALID;
+  }
   return SXC_OK;
 }
 
@@ -191,7 +196,7 @@ static std::vector<int2> build_work(int n, int chunk) {
 }
 
 // Lag-window form: zero-padded bf16 (hi, lo) or fp16 planes of every trace,
-// 32 x 32 pair tiles of the upper triangle, blocks of 8 lags.
+// 32 x 32 pair tiles of the upper triangle, blocks of kLagBlock lags.
 int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   sxc_plan* p = new sxc_plan();
   p->d = *desc;

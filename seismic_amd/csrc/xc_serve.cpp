@@ -35,6 +35,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -107,6 +108,7 @@ struct Request {
   Clock::time_point t_submit, t_done;
   bool done = false;
   int status = SXC_OK;
+  std::string error;  // the worker's message when status != SXC_OK
 };
 
 struct Slot {
@@ -329,6 +331,13 @@ void worker_loop(sxc_server* S, int slot_index) {
       }
     }
     const int rc = run_batch(S, s, groups);
+    std::string err;
+    if (rc != SXC_OK) {
+      err = last_error();
+      // launches and copies queued before the failing call may still use
+      // the slot's pinned buffers, which the next batch overwrites
+      (void)hipStreamSynchronize(s->stream);
+    }
     const auto now = Clock::now();
     {
       std::lock_guard<std::mutex> lk(S->mu);
@@ -344,6 +353,7 @@ void worker_loop(sxc_server* S, int slot_index) {
           }
         }
         q.status = rc;
+        q.error = err;
         q.t_done = now;
         q.done = true;
         if (rc == SXC_OK) {
@@ -525,7 +535,10 @@ int sxc_server_wait(sxc_server* S, long long request_id, sxc_match* out, double*
     S->cv_done.wait(lk, [&] { return q->done; });
     S->requests.erase(request_id);
   }
-  if (q->status != SXC_OK) { set_error("request %lld failed on the device", request_id); return q->status; }
+  if (q->status != SXC_OK) {
+    set_error("request %lld failed on the device: %s", request_id, q->error.c_str());
+    return q->status;
+  }
   if (out) memcpy(out, q->result.data(), q->result.size() * sizeof(sxc_match));
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count();
   return SXC_OK;

@@ -89,6 +89,10 @@ def test_invalid_plan_reported():
         engine.AllPairsPlan(4, 1, 100, 1024, 10, -10)
     with pytest.raises(RuntimeError, match="n_channels"):
         engine.AllPairsPlan(4, 0, 100, 1024, -10, 10)
+    # work items (row, 64-column chunk) number sum_c min(N, 64(c+1)) and are
+    # int-indexed: N = 524224 gives 2,147,221,504 items, N = 524225 2,147,745,729
+    with pytest.raises(RuntimeError, match="work-list limit"):
+        engine.AllPairsPlan(524225, 1, 100, 1024, -10, 10)
 
 
 def test_multicuda_binding_loads():
