@@ -204,7 +204,8 @@ int sxc_tiled_create(const sxc_plan_desc* desc, int block, int want_neg, sxc_til
 int sxc_tiled_start(sxc_tiled* t, const float* d_traces, long long trace_stride, int in_len);
 /* Waits for the next block and copies it into host buffers of row stride ld
  * elements (any pointer may be NULL to skip that output); info->index == -1
- * when the run is complete. */
+ * when the run is complete.  After a failed start or next the run is void:
+ * next returns SXC_ERR_INVALID until sxc_tiled_start succeeds again. */
 int sxc_tiled_next(sxc_tiled* t, sxc_block_info* info, unsigned short* coef_pos, short* lag_pos,
                    unsigned short* coef_neg, short* lag_neg, long long ld);
 /* pairs delivered so far (upper triangle incl. diagonal), near ties, overflow */

@@ -50,7 +50,7 @@ struct sxc_tiled {
   hipStream_t s_comp = nullptr, s_copy = nullptr;
   hipEvent_t e_packed[2] = {}, e_copied[2] = {};
   int delivered = 0, enqueued = 0;
-  bool started = false;
+  bool started = false;  // a run is in flight and every enqueue so far succeeded
   unsigned long long pairs = 0, near_ties = 0, tie_overflow = 0;
 };
 
@@ -263,7 +263,9 @@ int sxc_tiled_start(sxc_tiled* t, const float* d_traces, long long trace_stride,
     set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
     return SXC_ERR_INVALID;
   }
-  // a previous run still in flight is drained first
+  // a previous run still in flight is drained first; until this start has
+  // enqueued its first blocks, sxc_tiled_next reports no run
+  t->started = false;
   HIP_CHECK(hipStreamSynchronize(t->s_comp));
   HIP_CHECK(hipStreamSynchronize(t->s_copy));
   t->delivered = t->enqueued = 0;
@@ -285,7 +287,10 @@ int sxc_tiled_next(sxc_tiled* t, sxc_block_info* info, unsigned short* coef_pos,
                    unsigned short* coef_neg, short* lag_neg, long long ld) {
   clear_error();
   if (!t || !info) { set_error("null argument"); return SXC_ERR_INVALID; }
-  if (!t->started) { set_error("sxc_tiled_start has not been called"); return SXC_ERR_INVALID; }
+  if (!t->started) {
+    set_error("no run in flight (sxc_tiled_start was not called, or a launch of this run failed)");
+    return SXC_ERR_INVALID;
+  }
   const int n = (int)t->blocks.size();
   info->n_blocks = n;
   if (t->delivered >= n) {
@@ -329,7 +334,12 @@ int sxc_tiled_next(sxc_tiled* t, sxc_block_info* info, unsigned short* coef_pos,
   ++t->delivered;
   if (t->enqueued < n) {
     const int rc = enqueue_block(t, t->enqueued);
-    if (rc) return rc;
+    if (rc) {
+      // the failed block's events were not recorded: a later next() would
+      // wait on an older block's and deliver its data
+      t->started = false;
+      return rc;
+    }
     ++t->enqueued;
   }
   return SXC_OK;

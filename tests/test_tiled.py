@@ -82,6 +82,36 @@ def test_tiled_equals_packed(n, C, L, lo, hi, form, block, neg):
 
 
 @pytest.mark.gpu
+def test_tiled_run_state():
+    """next() before a start and after a rejected start reports that no run
+    is in flight; a second start over the same run delivers the same blocks."""
+    from seismic_amd.device import DeviceArray
+    n, L = 96, 256
+    ev, _ = synth.make_events(n, 1, L, seed=5)
+    print(f"case tiled run state n={n} L={L}", flush=True)
+    run = tiled.TiledRun(n, 1, L, -20, 20, form=engine.FORM_LAG_BF16X3, block=64)
+    tr = DeviceArray.from_numpy(ev)
+    try:
+        with pytest.raises(RuntimeError, match="no run in flight"):
+            run.next_block()
+        with pytest.raises(RuntimeError, match="in_len"):
+            run.start(tr, in_len=L + 1)
+        with pytest.raises(RuntimeError, match="no run in flight"):
+            run.next_block()
+        runs = []
+        for _ in range(2):
+            run.start(tr)
+            runs.append([(r[0]["index"], r[1].copy(), r[2].copy()) for r in run.blocks()])
+            assert run.next_block() is None   # complete: stays complete
+        assert len(runs[0]) == 3 and len(runs[1]) == 3
+        for a, b in zip(runs[0], runs[1]):
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    finally:
+        run.close()
+        tr.free()
+
+
+@pytest.mark.gpu
 def test_tiled_l4096_w1024_duplicates(oracle):
     """L=4096, window +-1024, bf16x3, log-uniform SNR in [0.3, 30] and 10%
     zero-filled time-shifted duplicates, N=640 in blocks of 256 (ragged)."""
