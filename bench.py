@@ -158,11 +158,13 @@ def lag_forms(ev, tr, stream, steps, warmup, f64=None):
     return out
 
 
-def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed_rates):
+def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed_rates,
+                alone_requests: int = 20):
     """Serving form: a 3-channel 2048-sample catalog resident in HBM; requests
     of 1..256 events with lengths U[512, 4096] at Poisson 200 req/s, top-10
-    matches each; then 64-event requests at a sweep of Poisson rates for the
-    per-request latency against load."""
+    matches each; then 64-event requests one at a time (latency of a request
+    alone on the GPU) and at a sweep of Poisson rates (latency against load).
+    A leg with 0 requests is skipped."""
     from seismic_amd import serving, synth
     log(f"bench: serving catalog {n_catalog} x 3 x 2048")
     cat, _ = synth.make_events(n_catalog, 3, 2048, seed=777)
@@ -194,17 +196,31 @@ def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed
                 "mean_requests_per_batch": round(st["mean_requests_per_batch"], 2),
                 "wall_s": round(r["wall_s"], 3)}
 
-    reqs = serving.make_requests(stream_requests, 3, (512, 4096), (1, 256), seed=11)
-    log(f"bench: serving stream, {stream_requests} requests at 200 req/s")
-    out["stream_1_256_events"] = leg(reqs, 200.0, 12)
-    reqs = serving.make_requests(fixed_requests, 3, (512, 4096), (64, 64), seed=13)
-    sweep = []
-    for rate in fixed_rates:
-        log(f"bench: serving 64-event requests, {fixed_requests} at {rate} req/s")
-        sweep.append(leg(reqs, rate, 14))
-    out["requests_64_events"] = sweep
-    ok = [r["offered_req_per_s"] for r in sweep if r["p99_ms"] < 50.0]
-    out["p99_below_50ms_up_to_req_per_s"] = max(ok) if ok else None
+    if stream_requests > 0:
+        reqs = serving.make_requests(stream_requests, 3, (512, 4096), (1, 256), seed=11)
+        log(f"bench: serving stream, {stream_requests} requests at 200 req/s")
+        out["stream_1_256_events"] = leg(reqs, 200.0, 12)
+    if alone_requests > 0:
+        reqs = serving.make_requests(alone_requests, 3, (512, 4096), (64, 64), seed=13)
+        log(f"bench: serving 64-event requests one at a time, {alone_requests}")
+        lat = []
+        for e, ln in reqs:
+            t0 = time.perf_counter()
+            srv.wait(srv.submit(e, ln))
+            lat.append((time.perf_counter() - t0) * 1e3)
+        lat = np.array(lat)
+        out["requests_64_events_alone"] = {"requests": alone_requests,
+                                           "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                                           "max_ms": round(float(lat.max()), 2)}
+    if fixed_requests > 0:
+        reqs = serving.make_requests(fixed_requests, 3, (512, 4096), (64, 64), seed=13)
+        sweep = []
+        for rate in fixed_rates:
+            log(f"bench: serving 64-event requests, {fixed_requests} at {rate} req/s")
+            sweep.append(leg(reqs, rate, 14))
+        out["requests_64_events"] = sweep
+        ok = [r["offered_req_per_s"] for r in sweep if r["p99_ms"] < 50.0]
+        out["p99_below_50ms_up_to_req_per_s"] = max(ok) if ok else None
     srv.close()
     return out
 
@@ -285,6 +301,8 @@ def main():
     ap.add_argument("--serve-catalog", type=int, default=32768)
     ap.add_argument("--serve-stream-requests", type=int, default=64)
     ap.add_argument("--serve-64-requests", type=int, default=200)
+    ap.add_argument("--serve-alone-requests", type=int, default=20,
+                    help="64-event requests submitted one at a time (latency alone on the GPU)")
     ap.add_argument("--serve-64-rates", type=str, default="5,10,20,30",
                     help="comma-separated Poisson rates (req/s) of the 64-event requests")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -359,7 +377,8 @@ def main():
     serve = None
     if not args.no_serving:
         serve = serving_leg(args.serve_catalog, args.serve_stream_requests, args.serve_64_requests,
-                            [float(r) for r in args.serve_64_rates.split(",") if r])
+                            [float(r) for r in args.serve_64_rates.split(",") if r],
+                            args.serve_alone_requests)
         log(f"bench: serving {json.dumps(serve)}")
 
     tiles = None
