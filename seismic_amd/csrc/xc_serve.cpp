@@ -18,12 +18,13 @@
 //
 // Lags.  Query q (length Lq) against catalog event c (length Lc) uses the FFT
 // period P: the smallest of 1024, 2048, 3072, 4096, 5120, 6144, 8192 with
-// P >= Lq + Lc - 1 and P/2 >= Lc, so the period-P correlation equals the
-// linear one on every lag that overlaps (-(Lc-1) .. Lq-1); lag > 0 means the
-// query is delayed relative to the catalog event.  Positive lags above P/2,
-// which the pair kernel reports as lag - P, are moved back when the results
-// are assembled (period_for, wrap_lag).  Events of one batch are grouped by
-// P and each group runs its own forward / pair / top-k launches.
+// P >= Lq + Lc - 1, so the period-P correlation equals the linear one on every
+// lag that overlaps (-(Lc-1) .. Lq-1); lag > 0 means the query is delayed
+// relative to the catalog event.  The pair kernel reports slot n as n or
+// n - P, whichever lies in (-P/2, P/2]; a reported lag outside
+// -(Lc-1) .. Lq-1 is moved back by P when the results are assembled
+// (period_for, wrap_lag).  Events of one batch are grouped by P and each group
+// runs its own forward / pair / top-k launches.
 #include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <string.h>
@@ -60,16 +61,22 @@ using Clock = std::chrono::steady_clock;
 constexpr int kPeriods[] = {1024, 2048, 3072, 4096, 5120, 6144, 8192};
 
 // Smallest FFT period whose circular correlation equals the linear one on
-// every overlapping lag -(lc-1) .. lq-1 (P >= lq + lc - 1).  The pair kernel
-// reports slot n as lag n <= P/2 ? n : n - P; P/2 >= lc keeps every negative
-// lag in the upper half of the slots, so it comes out right, and a positive
-// lag above P/2 comes out as lag - P < -(lc-1), where wrap_lag moves it back.
+// every overlapping lag -(lc-1) .. lq-1 (P >= lq + lc - 1).
 int period_for(int lq, int lc) {
   for (int P : kPeriods)
-    if (P >= lq + lc - 1 && P / 2 >= lc) return P;
+    if (P >= lq + lc - 1) return P;
   return 0;
 }
-int wrap_lag(int lag, int P, int lc) { return lag < -(lc - 1) ? lag + P : lag; }
+// The pair kernel reports slot n = lag mod P as n <= P/2 ? n : n - P.  The
+// lq + lc - 1 <= P linear lags occupy distinct slots, so a reported value
+// outside -(lc-1) .. lq-1 is the true lag shifted by one period: a true lag
+// above P/2 comes out as lag - P < -(lc-1), a true lag at or below -P/2 as
+// lag + P > lq-1.
+int wrap_lag(int lag, int P, int lq, int lc) {
+  if (lag > lq - 1) return lag - P;
+  if (lag < -(lc - 1)) return lag + P;
+  return lag;
+}
 
 // FFT tables of one period P (forward, inverse, C2R packing, post-twiddle)
 struct Tables {
@@ -349,7 +356,7 @@ void worker_loop(sxc_server* S, int slot_index) {
           for (int k = 0; k < d.top_k; ++k) {
             const size_t src = (size_t)row_of[r][e] * d.top_k + k;
             q.result[(size_t)e * d.top_k + k] =
-                sxc_match{s->h_tv[src], wrap_lag(s->h_tl[src], P, d.catalog_len), s->h_ti[src]};
+                sxc_match{s->h_tv[src], wrap_lag(s->h_tl[src], P, q.lengths[e], d.catalog_len), s->h_ti[src]};
           }
         }
         q.status = rc;

@@ -52,9 +52,9 @@ def test_invalid_server_desc_rejected():
         serving.Server(cat, max_query_len=100, top_k=40)
     with pytest.raises(RuntimeError, match="memory budget"):
         serving.Server(cat, max_query_len=100, top_k=2, mem_budget=1000)
-    # P/2 >= catalog length: no supported period above 8192
+    # P >= max_query_len + catalog_len - 1: no supported period above 8192
     with pytest.raises(RuntimeError, match="FFT period above 8192"):
-        serving.Server(np.zeros((4, 1, 5000), np.float32), max_query_len=100, top_k=2)
+        serving.Server(np.zeros((4, 1, 5000), np.float32), max_query_len=3194, top_k=2)
 
 
 @pytest.mark.gpu
@@ -94,18 +94,23 @@ def test_serving_matches_f64():
 @pytest.mark.gpu
 def test_serving_periods_and_wrapped_lags():
     """Query lengths on both sides of every period boundary of a 1200-sample
-    catalog (P = 3072, 4096, 5120, 6144, 8192: P >= Lq + Lc - 1 and
-    P/2 >= Lc), and exact shifted catalog copies at lags beyond P/2 (reported
-    by the kernel as lag - P and moved back) and at negative lags."""
+    catalog (P = 2048, 3072, 4096, 5120, 6144, 8192: the smallest P >= Lq +
+    Lc - 1), and exact shifted catalog copies at lags beyond P/2 (reported by
+    the kernel as lag - P and moved back), at or below -P/2 (reported as
+    lag + P and moved back) and at negative lags."""
     cat, _ = synth.make_events(40, 2, 1200, seed=71)
-    srv = serving.Server(cat, max_query_len=6000, top_k=4, max_batch_events=16, n_slots=2)
+    # event 8 carries its energy in its last 200 samples, so a query holding
+    # only samples 1024.. of it still matches it first
+    cat[8, :, :1000] *= 0.05
+    srv = serving.Server(cat, max_query_len=6000, top_k=4, max_batch_events=24, n_slots=2)
     lens = np.array([1, 1000, 1873, 1874, 2897, 2898, 4945, 4946, 6000, 4900, 5500, 1800,
-                     3921, 3922, 3900], np.int32)
+                     3921, 3922, 3900, 849, 850, 849], np.int32)
     q, _ = synth.make_events(len(lens), 2, 6000, seed=72)
     for e in range(len(lens)):
         q[e, :, lens[e]:] = 0
     # query -> (catalog event, lag): wrapped lags at P = 6144, 8192 and 5120
-    copies = {9: (5, 3700), 10: (17, 4200), 11: (23, -300), 14: (31, 2650)}
+    # (above P/2) and at P = 2048 (-1024 = -P/2, reported by the kernel as +1024)
+    copies = {9: (5, 3700), 10: (17, 4200), 11: (23, -300), 14: (31, 2650), 17: (8, -1024)}
     for e, (c, lag) in copies.items():
         q[e] = 0
         if lag >= 0:
@@ -138,9 +143,10 @@ def test_serving_periods_and_wrapped_lags():
 
 @pytest.mark.gpu
 def test_serving_catalog_longer_than_queries():
-    """Catalog events of 3000 samples, queries of 1..1000: P/2 >= L_c alone
-    sets P = 6144, and negative lags down to -(L_c - 1) come out unwrapped
-    (exact partial copies of catalog events at lags -2500 and -10)."""
+    """Catalog events of 3000 samples, queries of 1..1000 (P = 3072 or 4096
+    < 2 L_c): negative lags down to -(L_c - 1) come out right, -2500 at
+    P = 4096 through the fix-up of a lag at or below -P/2 (exact partial
+    copies of catalog events at lags -2500 and -10)."""
     cat, _ = synth.make_events(30, 1, 3000, seed=81)
     srv = serving.Server(cat, max_query_len=1000, top_k=3, max_batch_events=8, n_slots=1)
     lens = np.array([1000, 1000, 700, 1, 400], np.int32)
