@@ -34,9 +34,16 @@ class Server:
         self._pending = {}
 
     def submit(self, events: np.ndarray, lengths) -> int:
-        """events: [n, n_channels, stride] float32; lengths: n sample counts."""
+        """events: [n, n_channels, stride] float32 ([n, stride] with one
+        channel); lengths: n sample counts."""
         ev = np.ascontiguousarray(events, dtype=np.float32)
-        ln = np.ascontiguousarray(lengths, dtype=np.int32)
+        if ev.ndim == 2 and self.n_channels == 1:
+            ev = ev[:, None, :]
+        if ev.ndim != 3 or ev.shape[1] != self.n_channels:
+            raise ValueError(f"events must be [n, {self.n_channels}, stride] (got {events.shape})")
+        ln = np.ascontiguousarray(lengths, dtype=np.int32).reshape(-1)
+        if ln.shape[0] != ev.shape[0]:
+            raise ValueError(f"{ln.shape[0]} lengths for {ev.shape[0]} events")
         rid = ctypes.c_longlong()
         rc = self.lib.sxc_server_submit(self.handle, ev.ctypes.data_as(c_float_p), ev.shape[-1],
                                         ln.ctypes.data_as(c_int_p), ev.shape[0], ctypes.byref(rid))
@@ -47,7 +54,9 @@ class Server:
 
     def wait(self, request_id: int):
         """Returns (values [n, k], lags [n, k], indices [n, k], latency_ms)."""
-        n = self._pending.pop(request_id)
+        n = self._pending.pop(request_id, None)
+        if n is None:
+            raise RuntimeError(f"unknown or already collected request {request_id}")
         out = (SxcMatch * (n * self.top_k))()
         lat = ctypes.c_double()
         rc = self.lib.sxc_server_wait(self.handle, request_id, out, ctypes.byref(lat))

@@ -58,6 +58,38 @@ def test_invalid_server_desc_rejected():
 
 
 @pytest.mark.gpu
+def test_submit_checks_shapes():
+    """The wrapper rejects events whose channel count or length list does not
+    match (the C ABI cannot see buffer sizes) and unknown request ids; one
+    channel may come as [n, stride]."""
+    print("case serving submit shape checks", flush=True)
+    cat, _ = synth.make_events(20, 2, 256, seed=31)
+    srv = serving.Server(cat, max_query_len=300, top_k=2, max_batch_events=8, n_slots=1)
+    try:
+        q, _ = synth.make_events(3, 3, 300, seed=32)
+        with pytest.raises(ValueError, match="events must be"):
+            srv.submit(q, [300, 300, 300])
+        with pytest.raises(ValueError, match="lengths for"):
+            srv.submit(q[:, :2], [300, 300])
+        with pytest.raises(RuntimeError, match="unknown"):
+            srv.wait(12345)
+        v, l, idx, _ = srv.wait(srv.submit(q[:, :2], [300, 200, 100]))
+        assert v.shape == (3, 2) and (idx >= 0).all()
+    finally:
+        srv.close()
+    cat1, _ = synth.make_events(20, 1, 256, seed=33)
+    srv = serving.Server(cat1, max_query_len=300, top_k=2, max_batch_events=8, n_slots=1)
+    try:
+        q1 = np.zeros((2, 300), np.float32)
+        q1[0, 10:266] = cat1[5, 0]
+        q1[1, :256] = cat1[7, 0]
+        v, l, idx, _ = srv.wait(srv.submit(q1, [300, 256]))
+        assert idx[0, 0] == 5 and l[0, 0] == 10 and idx[1, 0] == 7 and l[1, 0] == 0
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
 def test_serving_matches_f64():
     cat, _ = synth.make_events(150, 3, 512, seed=21)
     srv = serving.Server(cat, max_query_len=900, top_k=5, max_batch_events=32, n_slots=2)
