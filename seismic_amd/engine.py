@@ -65,6 +65,12 @@ class AllPairsPlan:
         if out is None:
             out = self.alloc_outputs()
         vp, lp, vn, ln = out
+        # the kernels write n_pairs entries into each output: a smaller
+        # buffer would be written out of bounds on the device
+        for o, dt in zip(out, (np.float32, np.int32, np.float32, np.int32)):
+            if o.dtype != dt or o.nbytes < self.n_pairs * 4:
+                raise ValueError(f"outputs must be {self.n_pairs}-entry float32/int32/float32/int32 "
+                                 f"device arrays (got {o.dtype} of {o.nbytes} bytes)")
         sh = stream.handle if stream is not None else 0
         rc = self.lib.sxc_plan_execute(self.handle, traces.ptr, stride, in_len,
                                        vp.ptr, lp.ptr, vn.ptr, ln.ptr, ctypes.c_void_p(sh))

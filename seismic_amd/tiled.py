@@ -46,10 +46,19 @@ class TiledRun:
         `out`: optional arrays of at least [edge, edge] to fill (views of
         them are returned), edge = min(block, n_events)."""
         info = SxcBlockInfo()
+        k = 4 if self.want_neg else 2
         if out is None:
             e = self.edge
-            k = 4 if self.want_neg else 2
             out = [np.empty((e, e), np.float16 if q % 2 == 0 else np.int16) for q in range(k)]
+        # the library writes up to edge rows of row stride ld into each array
+        if len(out) < k or any(
+                o.ndim != 2 or not o.flags.c_contiguous or o.shape[0] < self.edge
+                or o.shape[1] != out[0].shape[1] or o.shape[1] < self.edge
+                or o.dtype != (np.float16 if q % 2 == 0 else np.int16)
+                for q, o in enumerate(out[:k])):
+            raise ValueError(f"out must hold {k} C-contiguous arrays of at least [{self.edge}, {self.edge}], "
+                             "float16 coefficients and int16 lags alternating, one row stride")
+        out = list(out[:k])
         ld = out[0].shape[1]
         ptrs = [o.ctypes.data_as(ctypes.c_void_p) for o in out]
         if not self.want_neg:

@@ -129,6 +129,37 @@ def test_multicuda_matches_cuda():
         assert np.array_equal(u, v)
 
 
+def test_execute_rejects_small_outputs():
+    """Caller-provided output buffers shorter than N(N+1)/2 entries, or of the
+    wrong type, are refused before any launch."""
+    from seismic_amd import engine
+    from seismic_amd.device import DeviceArray
+    print("case execute output checks", flush=True)
+    ev, _ = synth.make_events(40, 1, 500, seed=9)
+    plan = engine.AllPairsPlan(40, 1, 500, 1024, -100, 100)
+    tr = DeviceArray.from_numpy(ev)
+    n = plan.n_pairs
+    bufs = [DeviceArray(n - 1, np.float32), DeviceArray(n, np.int32),
+            DeviceArray(n, np.float32), DeviceArray(n, np.int32)]
+    try:
+        with pytest.raises(ValueError, match="outputs must be"):
+            plan.execute(tr, out=bufs)
+        bad_type = [DeviceArray(n, np.float32)] * 4
+        with pytest.raises(ValueError, match="outputs must be"):
+            plan.execute(tr, out=bad_type)
+        bad_type[0].free()
+        ok = plan.alloc_outputs()
+        plan.execute(tr, out=ok)
+        assert ok[0].numpy().shape == (n,)
+        for o in ok:
+            o.free()
+    finally:
+        for b in bufs:
+            b.free()
+        tr.free()
+        plan.close()
+
+
 def test_initialise_runtime():
     correlation_lib.initialiseRunTime()
 

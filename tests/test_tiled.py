@@ -98,6 +98,12 @@ def test_tiled_run_state():
             run.start(tr, in_len=L + 1)
         with pytest.raises(RuntimeError, match="no run in flight"):
             run.next_block()
+        run.start(tr)
+        small = [np.empty((32, 64), np.float16), np.empty((32, 64), np.int16)]
+        with pytest.raises(ValueError, match="out must hold"):
+            run.next_block(small)              # rows < edge: rejected before the call
+        with pytest.raises(ValueError, match="out must hold"):
+            run.next_block([np.empty((64, 64), np.float32), np.empty((64, 64), np.int16)])
         runs = []
         for _ in range(2):
             run.start(tr)
