@@ -357,6 +357,11 @@ def main():
                 "compute": "FP32 vector ALU (no MFMA); peak is MI355X FP32 157.3 TFLOP/s",
                 "flops_per_pair": fpp, "pairs_per_launch": int(packed),
                 "kernel_ms": round(tm["ms_pairs"], 4)}
+    if traffic:
+        # profiled HBM bytes of one launch over this run's kernel time
+        gbs = traffic / (tm["ms_pairs"] / 1e3) / 1e9
+        roofline["hbm_gb_per_s"] = round(gbs, 1)
+        roofline["hbm_frac"] = round(gbs / PEAK_HBM_GBS, 4)
 
     # accuracy: every form against an FP64 numpy reference on seeded pairs
     log(f"bench: FP64 reference on {args.fp64_pairs} sampled pairs")
