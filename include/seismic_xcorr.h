@@ -77,7 +77,10 @@ int sxc_plan_destroy(sxc_plan* plan);
  *   d_traces: [n_events][n_channels][trace_stride] float32 on the device;
  *             samples at index >= in_len are taken as zero (in_len <= P).
  *   outputs:  device arrays of n_events*(n_events+1)/2 entries.
- *   stream:   hipStream_t (NULL = default stream); the call is asynchronous. */
+ *   stream:   hipStream_t (NULL = default stream); the call is asynchronous.
+ * Different plans share no device state and may run on concurrent streams;
+ * executes of one plan share its workspace, so they must be ordered (one
+ * stream, or a synchronisation between streams). */
 int sxc_plan_execute(sxc_plan* plan, const float* d_traces, long long trace_stride, int in_len,
                      float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
                      void* stream);

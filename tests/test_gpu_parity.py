@@ -160,6 +160,48 @@ def test_execute_rejects_small_outputs():
         plan.close()
 
 
+def test_plans_on_concurrent_streams():
+    """Three plans (two FFT periods and the bf16x3 lag form) launched on three
+    streams at once give bitwise the results of sequential runs on the
+    default stream: plans share no device state."""
+    from seismic_amd import engine
+    from seismic_amd.device import DeviceArray, Stream, synchronize
+    print("case plans on 3 concurrent streams", flush=True)
+    cfgs = [(200, 1, 900, 2048, -500, 500, engine.FORM_FFT_F32),
+            (150, 3, 1500, 4096, -1200, 1300, engine.FORM_FFT_F32),
+            (180, 2, 700, 0, -100, 100, engine.FORM_LAG_BF16X3)]
+    plans, trs, seq = [], [], []
+    for k, (n, c, L, P, lo, hi, form) in enumerate(cfgs):
+        ev, _ = synth.make_events(n, c, L, seed=40 + k)
+        plans.append(engine.AllPairsPlan(n, c, L, P, lo, hi, form=form))
+        trs.append(DeviceArray.from_numpy(ev))
+    try:
+        for pl, tr in zip(plans, trs):
+            outs = pl.execute(tr)
+            synchronize()
+            seq.append([o.numpy() for o in outs])
+            for o in outs:
+                o.free()
+        streams = [Stream() for _ in plans]
+        outs = [pl.alloc_outputs() for pl in plans]
+        for _ in range(2):
+            for pl, tr, o, st in zip(plans, trs, outs, streams):
+                pl.execute(tr, out=o, stream=st)
+        for st in streams:
+            st.synchronize()
+        for k in range(len(plans)):
+            for a, b in zip(seq[k], [o.numpy() for o in outs[k]]):
+                assert np.array_equal(a, b), k
+        for st, oo in zip(streams, outs):
+            st.destroy()
+            for o in oo:
+                o.free()
+    finally:
+        for pl, tr in zip(plans, trs):
+            pl.close()
+            tr.free()
+
+
 def test_initialise_runtime():
     correlation_lib.initialiseRunTime()
 
