@@ -29,6 +29,11 @@
  *     the gap between the winning peak and the best value at any other lag,
  *     which the parity tests use to classify near-ties.  Pinned by
  *     tests/test_oracle.py against numpy FP64 FFT correlation and brute force.
+ *
+ * A third entry, oracle_allpairs_stacked_f32, runs the OMP algorithm with the
+ * channel stacking of the multi-channel workload (per-channel spectra
+ * products summed before one c2r); it is the reported CPU baseline of that
+ * workload, pinned by tests/test_oracle.py against the FP64 direct oracle.
  */
 #include <math.h>
 #include <stdint.h>
