@@ -114,3 +114,35 @@ def test_omp_binding_loads_and_rejects_bad_classes():
                        fp(outs[2]), ip(outs[3]), ip(hist[0]), ip(hist[1]))
     assert b"n_classes_hist" in lib.seismic_last_error()
     assert all((o == 7).all() for o in outs) and not hist[0].any()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror in seismic_amd._lib has the size and field offsets
+    of its C struct in include/seismic_xcorr.h (compiled with the host gcc)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"sxc_plan_desc": _lib.SxcPlanDesc, "sxc_stats": _lib.SxcStats,
+               "sxc_timing": _lib.SxcTiming, "sxc_server_desc": _lib.SxcServerDesc,
+               "sxc_match": _lib.SxcMatch, "sxc_serve_stats": _lib.SxcServerStats,
+               "sxc_block_info": _lib.SxcBlockInfo}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "seismic_xcorr.h"', "int main(void) {"]
+    for name, cls in mirrors.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'  printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for name, cls in mirrors.items():
+        assert got[(name, "size")] == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
+
