@@ -146,3 +146,43 @@ def test_struct_layouts_match_header(tmp_path):
         for f, _ in cls._fields_:
             assert got[(name, f)] == getattr(cls, f).offset, (name, f)
 
+
+# The reference's exported signatures, as its sources declare them
+# (Correlation_in_CUDA/correlation_cuda.cu:509,530-533,
+# Correlation_in_MultiCUDA/correlation_multicuda.cu:19,48-51,
+# Correlation_in_OMP/correlation_c.cpp:210-213).
+_REFERENCE_SIGNATURES = {
+    "correlation_cuda.h": [
+        "void (*f0)(void) = initialiseCUDA;",
+        "void (*f1)(float*, float*, int, int, int, int, int, float, float*, int*, float*, int*)"
+        " = correlationCUDA;"],
+    "correlation_multicuda.h": [
+        "void (*f0)(int) = initialiseCUDA;",
+        "void (*f1)(float*, float*, int, int, int, int, float, int, float*, int*, float*, int*)"
+        " = correlationCUDA;"],
+    "correlation_c.h": [
+        "void (*f1)(float*, float*, int, int, int, int, int, int, float, int, float*, int*, float*,"
+        " int*, int*, int*) = correlationCPP;"],
+}
+
+
+@pytest.mark.parametrize("header", sorted(_REFERENCE_SIGNATURES))
+def test_dropin_signatures_match_reference(header, tmp_path):
+    """Each drop-in header declares the reference's exact signature: taking
+    the function's address into a pointer of the reference type compiles
+    without a warning, as C and as C++."""
+    import shutil
+    import subprocess
+    for comp, std, ext in (("gcc", "-std=c99", "c"), ("g++", "-std=c++17", "cpp")):
+        exe = shutil.which(comp)
+        if exe is None:
+            pytest.skip(f"no {comp}")
+        body = "\n".join(f"  {l}" for l in _REFERENCE_SIGNATURES[header])
+        src = tmp_path / f"sig.{ext}"
+        src.write_text(f'#include "{header}"\nint main(void) {{\n{body}\n'
+                       f'  return (int)(sizeof(f1) == 0);\n}}\n')
+        r = subprocess.run([exe, std, "-Wall", "-Werror", "-Wno-unused-variable", "-I",
+                            os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "sig.o")],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
