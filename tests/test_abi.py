@@ -186,3 +186,34 @@ def test_dropin_signatures_match_reference(header, tmp_path):
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
 
+
+def _header_params(header, func):
+    """Parameter C types of `func` as `header` declares it."""
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", header)).read(), flags=re.S)
+    m = re.search(r"\b" + func + r"\s*\(([^)]*)\)\s*;", txt)
+    assert m, (header, func)
+    out = []
+    for p in m.group(1).split(","):
+        p = " ".join(p.split())
+        if p == "void":
+            continue
+        ptr = "*" in p
+        base = p.replace("*", " ").split()[0]
+        out.append(base + ("*" if ptr else ""))
+    return out
+
+
+@pytest.mark.parametrize("module,loader,header,func", [
+    ("correlation_lib", "initialise_library", "correlation_cuda.h", "correlationCUDA"),
+    ("multicuda_lib", "initialise_library", "correlation_multicuda.h", "correlationCUDA"),
+    ("omp_lib", "initialize_library", "correlation_c.h", "correlationCPP"),
+])
+def test_binding_argtypes_match_headers(module, loader, header, func):
+    """The Python bindings' ctypes argtypes are the headers' parameter types."""
+    import importlib
+    names = {ctypes.POINTER(ctypes.c_float): "float*", ctypes.POINTER(ctypes.c_int): "int*",
+             ctypes.c_int: "int", ctypes.c_float: "float"}
+    lib = getattr(importlib.import_module(f"seismic_amd.{module}"), loader)()
+    got = [names[t] for t in getattr(lib, func).argtypes]
+    assert got == _header_params(header, func)
+
