@@ -80,10 +80,18 @@ def vs_fp64(outs, idx, ref):
     """Coefficient error and lag agreement of packed outputs (vp, lp, vn, ln)
     against the FP64 reference on the sampled pairs."""
     vp, lp, vn, ln = outs
+    dp, dn = lp[idx] != ref["lag_pos"], ln[idx] != ref["lag_neg"]
+    agree = ~dp & ~dn
+    sep = (ref["gap_pos"] >= 1e-5) & (ref["gap_neg"] >= 1e-5)
+    gaps = np.concatenate([ref["gap_pos"][dp], ref["gap_neg"][dn]])
     return {"pairs": int(idx.size),
             "max_abs_dcoef": float(max(np.abs(vp[idx] - ref["pos"]).max(), np.abs(vn[idx] - ref["neg"]).max())),
-            "lag_agreement": float(((lp[idx] == ref["lag_pos"]) & (ln[idx] == ref["lag_neg"])).mean()),
-            "fp64_near_ties_below_1e-5": int(((ref["gap_pos"] < 1e-5) | (ref["gap_neg"] < 1e-5)).sum())}
+            "lag_agreement": float(agree.mean()),
+            "fp64_near_ties_below_1e-5": int((~sep).sum()),
+            # pairs whose top two FP64 values differ by at least 1e-5 on both sides
+            "lag_agreement_separated": float(agree[sep].mean()) if sep.any() else None,
+            # largest FP64 top-two gap on a side whose lag differs (None: none differ)
+            "disagreeing_side_max_fp64_gap": float(gaps.max()) if gaps.size else None}
 
 
 def time_plan(plan, tr, outs, stream, steps, warmup):
