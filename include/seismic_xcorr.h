@@ -34,6 +34,9 @@ enum {
   SXC_ERR_INVALID = -1,   /* bad argument or unsupported shape */
   SXC_ERR_HIP = -2,       /* a HIP runtime call failed */
   SXC_ERR_MEMORY = -3,    /* device allocation failed */
+  SXC_ERR_TIES = -4,      /* results written, but near ties overflowed the FP64 resolver's
+                             record capacity: those pairs keep their FP32 argmax lag */
+  SXC_ERR_OVERLOAD = -5,  /* serving: request shed by admission (latency budget or queue bound) */
 };
 
 enum {
@@ -84,6 +87,22 @@ int sxc_plan_destroy(sxc_plan* plan);
 int sxc_plan_execute(sxc_plan* plan, const float* d_traces, long long trace_stride, int in_len,
                      float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
                      void* stream);
+
+/* The reference's own formulation, for callers that hand over reversed
+ * traces as the reference bindings do (Correlation_in_CUDA/correlation_lib.py:59-62):
+ * the column operand of pair (i, j) is the spectrum of the caller's row
+ * d_reversed[j], as in X_i * R_j (Correlation_in_OMP/correlation_c.cpp:43-45,
+ * :270), and lag l is that product's inverse transform at index
+ * trace_len - 1 + l (:102-103).  For r_j the flip of an event of its own
+ * length L_j this is the correlation at true lag l + trace_len - L_j, the
+ * reference's result on events of uneven length; with every L_j = trace_len
+ * it equals sxc_plan_execute.  Norms are those of d_traces' first trace_len
+ * samples (:195-204).  FFT form only, demean = 0; d_reversed has the layout
+ * of d_traces.  The near-tie resolver sums x_i[m] r_j[(trace_len - 1 + l - m) mod P]
+ * in FP64. */
+int sxc_plan_execute_reversed(sxc_plan* plan, const float* d_traces, const float* d_reversed,
+                              long long trace_stride, int in_len, float* d_vals_pos, int* d_lags_pos,
+                              float* d_vals_neg, int* d_lags_neg, void* stream);
 
 /* Counters of the last execute on this plan (synchronises its stream). */
 int sxc_plan_stats(sxc_plan* plan, sxc_stats* out);
@@ -159,6 +178,11 @@ typedef struct {
   double mean_requests_per_batch;
   double events_per_s;               /* completed events / (last completion - first submit) */
   unsigned long long slot_bytes;     /* device memory reserved per slot */
+  unsigned long long offered;        /* submits since the last reset (admitted + rejected) */
+  unsigned long long rejected;       /* submits shed by admission (SXC_ERR_OVERLOAD) */
+  double budget_ms;                  /* latency budget of the admission (0: off) */
+  double model_tflops;               /* modelled device rate, algorithmic FFT-form TFLOP/s */
+  double model_overhead_ms;          /* modelled fixed cost per request */
 } sxc_serve_stats;
 
 /* catalog: host [n_catalog][n_channels][catalog_len] float32 (copied; the
@@ -171,6 +195,13 @@ int sxc_server_submit(sxc_server* server, const float* events, long long stride,
 /* Blocks until the request is done; out: [n_events][top_k] matches. */
 int sxc_server_wait(sxc_server* server, long long request_id, sxc_match* out, double* latency_ms);
 int sxc_server_stats(sxc_server* server, sxc_serve_stats* out);
+/* Deadline admission: from now on a submit whose predicted latency (the
+ * modelled device work already admitted plus its own, plus a fixed
+ * per-request cost) exceeds budget_ms returns SXC_ERR_OVERLOAD and is not
+ * queued, unless the device is idle; 0 turns it off (the default).  The FIFO
+ * holds at most 4096 waiting requests either way.  Requests shed this way can
+ * be submitted again later. */
+int sxc_server_set_latency_budget(sxc_server* server, double budget_ms);
 int sxc_server_reset_stats(sxc_server* server);
 /* Serves every request still queued, then frees the server.  No other thread
  * may be inside a call on this server (e.g. sxc_server_wait) meanwhile. */
@@ -220,6 +251,10 @@ int sxc_tiled_destroy(sxc_tiled* t);
  * outputs back.  traces: [n_events][n_channels][trace_stride] float32. */
 int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,
                       int in_len, float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg);
+/* The same with the caller's reversed rows (see sxc_plan_execute_reversed). */
+int sxc_allpairs_host_reversed(const sxc_plan_desc* desc, const float* traces, const float* reversed,
+                               long long trace_stride, int in_len, float* vals_pos, int* lags_pos,
+                               float* vals_neg, int* lags_neg);
 
 #ifdef __cplusplus
 }

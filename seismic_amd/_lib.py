@@ -77,6 +77,11 @@ class SxcServerStats(ctypes.Structure):
         ("mean_requests_per_batch", ctypes.c_double),
         ("events_per_s", ctypes.c_double),
         ("slot_bytes", ctypes.c_ulonglong),
+        ("offered", ctypes.c_ulonglong),
+        ("rejected", ctypes.c_ulonglong),
+        ("budget_ms", ctypes.c_double),
+        ("model_tflops", ctypes.c_double),
+        ("model_overhead_ms", ctypes.c_double),
     ]
 
 
@@ -88,13 +93,15 @@ class SxcBlockInfo(ctypes.Structure):
 # every symbol include/*.h declares, per library
 EXPORTS = {
     "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
-                            "sxc_plan_execute", "sxc_plan_stats", "sxc_allpairs_host",
+                            "sxc_plan_execute", "sxc_plan_execute_reversed", "sxc_plan_stats",
+                            "sxc_allpairs_host", "sxc_allpairs_host_reversed",
                             "sxc_plan_enable_timing", "sxc_plan_timing", "sxc_device_count",
                             "sxc_device_alloc", "sxc_device_free", "sxc_memcpy_h2d",
                             "sxc_memcpy_d2h", "sxc_memset_d8", "sxc_stream_create",
                             "sxc_stream_destroy", "sxc_stream_synchronize",
                             "sxc_device_synchronize", "sxc_copy_kernel", "sxc_server_create",
                             "sxc_server_submit", "sxc_server_wait", "sxc_server_stats",
+                            "sxc_server_set_latency_budget",
                             "sxc_server_reset_stats", "sxc_server_destroy", "sxc_tiled_create",
                             "sxc_tiled_start", "sxc_tiled_next", "sxc_tiled_stats",
                             "sxc_tiled_destroy"],
@@ -129,6 +136,9 @@ def load_engine() -> ctypes.CDLL:
     lib.sxc_plan_destroy.argtypes = [vp]
     lib.sxc_plan_execute.restype = ctypes.c_int
     lib.sxc_plan_execute.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, vp, vp]
+    lib.sxc_plan_execute_reversed.restype = ctypes.c_int
+    lib.sxc_plan_execute_reversed.argtypes = [vp, vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, vp,
+                                              vp]
     lib.sxc_plan_stats.restype = ctypes.c_int
     lib.sxc_plan_stats.argtypes = [vp, ctypes.POINTER(SxcStats)]
     lib.sxc_plan_enable_timing.restype = ctypes.c_int
@@ -160,6 +170,8 @@ def load_engine() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_double)]
     lib.sxc_server_stats.restype = ctypes.c_int
     lib.sxc_server_stats.argtypes = [vp, ctypes.POINTER(SxcServerStats)]
+    lib.sxc_server_set_latency_budget.restype = ctypes.c_int
+    lib.sxc_server_set_latency_budget.argtypes = [vp, ctypes.c_double]
     lib.sxc_server_reset_stats.restype = ctypes.c_int
     lib.sxc_server_reset_stats.argtypes = [vp]
     lib.sxc_server_destroy.restype = ctypes.c_int
@@ -178,6 +190,10 @@ def load_engine() -> ctypes.CDLL:
     lib.sxc_allpairs_host.restype = ctypes.c_int
     lib.sxc_allpairs_host.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, ctypes.c_longlong,
                                       ctypes.c_int, c_float_p, c_int_p, c_float_p, c_int_p]
+    lib.sxc_allpairs_host_reversed.restype = ctypes.c_int
+    lib.sxc_allpairs_host_reversed.argtypes = [ctypes.POINTER(SxcPlanDesc), c_float_p, c_float_p,
+                                               ctypes.c_longlong, ctypes.c_int, c_float_p, c_int_p,
+                                               c_float_p, c_int_p]
     _engine = lib
     return lib
 

@@ -13,9 +13,10 @@ void correlationCUDA(float* events, float* events_reversed, int n_events, int ev
                      int shift, int paddedSize, int num_threads, float threshold,
                      float* xcorr_vals_pos, int* xcorr_lags_pos, float* xcorr_vals_neg,
                      int* xcorr_lags_neg) {
-  (void)events_reversed;  // the engine derives the reversed spectrum from the forward one
-  (void)num_threads;      // printed, never used, by the reference (correlation_cuda.cu:565)
-  sxc_reference_call(events, n_events, event_length, shift, paddedSize, threshold,
+  (void)num_threads;  // printed, never used, by the reference (correlation_cuda.cu:565)
+  // events_reversed is transformed like the reference's second R2C (:619-636)
+  // and forms X_i * R_j, so events of uneven length give the reference's lags
+  sxc_reference_call(events, events_reversed, n_events, event_length, shift, paddedSize, threshold,
                      xcorr_vals_pos, xcorr_lags_pos, xcorr_vals_neg, xcorr_lags_neg);
 }
 

@@ -17,9 +17,8 @@ void initialiseCUDA(int nGPUS) {
 void correlationCUDA(float* events, float* events_reversed, int n_events, int event_length,
                      int shift, int paddedSize, float threshold, int nGPUS, float* xcorr_vals_pos,
                      int* xcorr_lags_pos, float* xcorr_vals_neg, int* xcorr_lags_neg) {
-  (void)events_reversed;
   (void)nGPUS;
-  sxc_reference_call(events, n_events, event_length, shift, paddedSize, threshold,
+  sxc_reference_call(events, events_reversed, n_events, event_length, shift, paddedSize, threshold,
                      xcorr_vals_pos, xcorr_lags_pos, xcorr_vals_neg, xcorr_lags_neg);
 }
 

@@ -77,6 +77,15 @@ int validate_desc(const sxc_plan_desc& d) {
       return SXC_ERR_INVALID;
     }
     if (!(d.threshold == d.threshold)) { set_error("threshold is NaN"); return SXC_ERR_INVALID; }
+    // whole blocks of kLagBlock lags from lag_first (== 1 mod 8, <= lag_lo) must fit kLagMaxSpan
+    long long lf = d.lag_lo;
+    while (((lf - 1) % 8 + 8) % 8 != 0) --lf;
+    const long long span = (d.lag_hi - lf + 1 + kLagBlock - 1) / kLagBlock * kLagBlock;
+    if (span > kLagMaxSpan) {
+      set_error("lag window [%d, %d] spans more than %d lags (the lag forms' limit)", d.lag_lo, d.lag_hi,
+                kLagMaxSpan - 16);
+      return SXC_ERR_INVALID;
+    }
     return SXC_OK;
   }
   if (!fft_period_supported(P)) {
@@ -108,7 +117,7 @@ using namespace sxc;
 void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
-  void* bufs[] = {p->spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
+  void* bufs[] = {p->spec, p->col_spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
                   p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -160,16 +169,23 @@ int slot_lag_host(int R, int lane, int slot) {
 
 }  // namespace sxc
 
-static int upload_tables(sxc_plan* p) {
-  const int R = p->R, M = p->M, P = p->P;
-  std::vector<float2> fw = wave_tables(R, -1), iw = wave_tables(R, +1), post(M + 1);
-  for (int k = 0; k <= M; ++k) twiddle(k, P, -1, &post[k].x, &post[k].y);
+// Window lane masks of an FFT-form plan: bit l of mask s is set when slot s
+// of lane l holds a lag inside [lag_lo, lag_hi].
+std::vector<unsigned long long> sxc::window_masks(int R, int lag_lo, int lag_hi) {
   std::vector<unsigned long long> wm(2 * R, 0ull);
   for (int s = 0; s < 2 * R; ++s)
     for (int l = 0; l < 64; ++l) {
       const int lag = slot_lag_host(R, l, s);
-      if (lag >= p->d.lag_lo && lag <= p->d.lag_hi) wm[s] |= 1ull << l;
+      if (lag >= lag_lo && lag <= lag_hi) wm[s] |= 1ull << l;
     }
+  return wm;
+}
+
+static int upload_tables(sxc_plan* p) {
+  const int R = p->R, M = p->M, P = p->P;
+  std::vector<float2> fw = wave_tables(R, -1), iw = wave_tables(R, +1), post(M + 1);
+  for (int k = 0; k <= M; ++k) twiddle(k, P, -1, &post[k].x, &post[k].y);
+  std::vector<unsigned long long> wm = window_masks(R, p->d.lag_lo, p->d.lag_hi);
   int rc;
   if ((rc = dev_alloc(&p->fwd2, fw.size(), "twiddles"))) return rc;
   if ((rc = dev_alloc(&p->inv2, iw.size(), "twiddles"))) return rc;
@@ -323,9 +339,113 @@ static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_
   return SXC_OK;
 }
 
+// Near ties beyond the record capacity keep their FP32 argmax lag; a caller
+// of the host wrappers is told through the error string.
+static int check_tie_overflow(sxc_plan* p) {
+  sxc_stats stt{};
+  int rc = sxc_plan_stats(p, &stt);
+  if (rc) return rc;
+  if (stt.tie_overflow) {
+    set_error("%llu near ties exceeded the FP64 resolver's record capacity (%d): their lags are the FP32 "
+              "argmax, not the FP64 one",
+              stt.tie_overflow, p->tie_cap);
+    return SXC_ERR_TIES;
+  }
+  return SXC_OK;
+}
+
+// Host-memory run: copy traces (and reversed rows) in, execute, check, copy
+// the packed outputs back, free.  The outputs are written even when near ties
+// overflowed (the error is still returned).
+static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const float* reversed,
+                         long long trace_stride, int in_len, float* vals_pos, int* lags_pos, float* vals_neg,
+                         int* lags_neg) {
+  if (!desc || !traces || !vals_pos || !lags_pos || !vals_neg || !lags_neg) {
+    set_error("null argument");
+    return SXC_ERR_INVALID;
+  }
+  sxc_plan* p = nullptr;
+  int rc = sxc_plan_create(desc, &p);
+  if (rc) return rc;
+  const long long rows = (long long)desc->n_events * desc->n_channels;
+  const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
+  const size_t np = (size_t)p->n_pairs;
+  float *d_in = nullptr, *d_rev = nullptr, *d_vp = nullptr, *d_vn = nullptr;
+  int *d_lp = nullptr, *d_ln = nullptr;
+  auto cleanup = [&]() {
+    void* bufs[] = {d_in, d_rev, d_vp, d_vn, d_lp, d_ln};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    sxc_plan_destroy(p);
+  };
+  if ((rc = dev_alloc(&d_in, (size_t)(rows * trace_stride), "input traces")) ||
+      (reversed && (rc = dev_alloc(&d_rev, (size_t)(rows * trace_stride), "reversed traces"))) ||
+      (rc = dev_alloc(&d_vp, np, "output values")) || (rc = dev_alloc(&d_vn, np, "output values")) ||
+      (rc = dev_alloc(&d_lp, np, "output lags")) || (rc = dev_alloc(&d_ln, np, "output lags"))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpy(d_in, traces, in_bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess && reversed) e = hipMemcpy(d_rev, reversed, in_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("copying traces to the device failed: %s", hipGetErrorString(e));
+    cleanup();
+    return SXC_ERR_HIP;
+  }
+  rc = reversed ? sxc_plan_execute_reversed(p, d_in, d_rev, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr)
+                : sxc_plan_execute(p, d_in, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr);
+  if (rc == SXC_OK) {
+    hipError_t es = hipDeviceSynchronize();
+    if (es != hipSuccess) {
+      set_error("device execution failed: %s", hipGetErrorString(es));
+      rc = SXC_ERR_HIP;
+    }
+  }
+  int tie_rc = SXC_OK;
+  if (rc == SXC_OK) tie_rc = check_tie_overflow(p);
+  if (rc == SXC_OK) {
+    hipError_t e1 = hipMemcpy(vals_pos, d_vp, np * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(lags_pos, d_lp, np * sizeof(int), hipMemcpyDeviceToHost);
+    hipError_t e3 = hipMemcpy(vals_neg, d_vn, np * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t e4 = hipMemcpy(lags_neg, d_ln, np * sizeof(int), hipMemcpyDeviceToHost);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+      set_error("copying results to the host failed");
+      rc = SXC_ERR_HIP;
+    }
+  }
+  cleanup();
+  return rc ? rc : tie_rc;
+}
+
 extern "C" {
 
 const char* seismic_last_error(void) { return last_error(); }
+
+}  // extern "C"
+
+// Geometry of an FFT-form plan (no device memory): period, strides, pair and
+// work-item counts, near-tie band and record capacity.
+void sxc::fft_plan_geometry(const sxc_plan_desc& desc, long long tie_cap_max, sxc_plan* p) {
+  p->d = desc;
+  p->P = desc.padded_size;
+  p->M = p->P / 2;
+  p->R = p->M / 64;
+  p->full = (desc.lag_lo == -p->M + 1 && desc.lag_hi == p->M);
+  p->spec_stride = p->M + 32;  // 256-byte aligned rows, bins 0..M
+  p->n_pairs = (long long)desc.n_events * (desc.n_events + 1) / 2;
+  // default band: 4x the largest FP32-vs-FP64 coefficient error measured
+  // (5e-7); any band of at least twice the error leaves every pair outside it
+  // with the FP64 winner (a flipped winner has an FP32 gap below 2x the error)
+  p->tie_tol = desc.tie_tol > 0.f ? desc.tie_tol : 2e-6f;
+  p->tie_cap = (int)std::min<long long>(p->n_pairs, tie_cap_max);
+  // work items (row, 64-column chunk), chunk-major: chunk c holds min(N, 64 (c + 1)) rows
+  long long items = 0;
+  for (long long j1 = p->chunk; j1 - p->chunk < desc.n_events; j1 += p->chunk)
+    items += std::min<long long>(j1, desc.n_events);
+  p->n_work = (int)items;
+}
+
+extern "C" {
 
 int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   clear_error();
@@ -335,18 +455,7 @@ int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   if (rc) return rc;
   if (is_lag_form(desc->form)) return lag_plan_create(desc, out);
   sxc_plan* p = new sxc_plan();
-  p->d = *desc;
-  p->P = desc->padded_size;
-  p->M = p->P / 2;
-  p->R = p->M / 64;
-  p->full = (desc->lag_lo == -p->M + 1 && desc->lag_hi == p->M);
-  p->spec_stride = p->M + 32;  // 256-byte aligned rows, bins 0..M
-  p->n_pairs = (long long)desc->n_events * (desc->n_events + 1) / 2;
-  // default band: 4x the largest FP32-vs-FP64 coefficient error measured
-  // (5e-7); any band of at least twice the error leaves every pair outside it
-  // with the FP64 winner (a flipped winner has an FP32 gap below 2x the error)
-  p->tie_tol = desc->tie_tol > 0.f ? desc->tie_tol : 2e-6f;
-  p->tie_cap = (int)std::min<long long>(p->n_pairs, 1LL << 24);
+  fft_plan_geometry(*desc, 1LL << 24, p);
   HIP_CHECK(hipGetDevice(&p->device));
   const long long rows = (long long)desc->n_events * desc->n_channels;
   auto fail = [&](int code) { plan_free(p); return code; };
@@ -354,7 +463,7 @@ int sxc_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   if ((rc = dev_alloc(&p->norms, (size_t)rows, "norms"))) return fail(rc);
   if (desc->demean && (rc = dev_alloc(&p->traces, (size_t)(rows * p->P), "demeaned traces"))) return fail(rc);
   std::vector<int2> work = build_work(desc->n_events, p->chunk);
-  p->n_work = (int)work.size();
+  if ((int)work.size() != p->n_work) { set_error("work list size mismatch"); return fail(SXC_ERR_INVALID); }
   if ((rc = dev_alloc(&p->work, work.size(), "work list"))) return fail(rc);
   if (hipMemcpy(p->work, work.data(), work.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("copying the work list failed");
@@ -372,28 +481,17 @@ int sxc_plan_destroy(sxc_plan* plan) {
   return SXC_OK;
 }
 
-int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
-                     float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
-                     void* stream) {
-  clear_error();
-  if (!p || !d_traces || !d_vals_pos || !d_lags_pos || !d_vals_neg || !d_lags_neg) {
-    set_error("null argument");
-    return SXC_ERR_INVALID;
-  }
-  if (!p->lag && (in_len < 1 || in_len > p->P || trace_stride < in_len)) {
-    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
-    return SXC_ERR_INVALID;
-  }
-  hipStream_t st = (hipStream_t)stream;
-  p->last_stream = st;
-  if (p->lag) {
-    if (in_len > trace_stride || in_len < 1) {
-      set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
-      return SXC_ERR_INVALID;
-    }
-    return lag_plan_execute(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,
-                            d_lags_neg, st);
-  }
+}  // extern "C"
+
+// FFT form.  d_rev == nullptr: column spectra are the row spectra (lag l of
+// pair (i, j) is sum_n x_i[n] x_j[n - l], circular mod P).  d_rev != nullptr:
+// the reference's formulation X_i R_j with the caller's reversed rows
+// (Correlation_in_OMP/correlation_c.cpp:43-45, :270) -- their spectra,
+// rotated by L - 1, become the column spectra, so lag l reads the reference's
+// corr_t[L - 1 + l] whatever length each event really has.
+int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride,
+                          int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                          hipStream_t st) {
   const sxc_plan_desc& d = p->d;
   const int rows = d.n_events * d.n_channels;
 
@@ -416,11 +514,24 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
   if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
   HIP_CHECK(launch_forward(p->R, fa, st));
+  if (d_rev) {
+    ForwardArgs ra = fa;
+    ra.in = d_rev;
+    ra.demean = 0;
+    ra.norms = nullptr;
+    ra.out_traces = nullptr;
+    ra.spec = p->col_spec;
+    ra.reversed = 1;
+    ra.rev_shift = d.trace_len - 1;
+    ra.norms_in = p->norms;
+    HIP_CHECK(launch_forward(p->R, ra, st));
+  }
   HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
 
   PairArgs pa{};
   pa.spec = p->spec;
+  pa.col_spec = d_rev ? p->col_spec : nullptr;
   pa.spec_stride = p->spec_stride;
   pa.n_events = d.n_events;
   pa.n_channels = d.n_channels;
@@ -448,6 +559,13 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   ResolveArgs ra{};
   ra.spec = p->spec;
   ra.spec_stride = p->spec_stride;
+  if (d_rev) {
+    ra.col_spec = p->col_spec;
+    ra.rev_traces = d_rev;
+    ra.rev_stride = trace_stride;
+    ra.rev_len = in_len;
+    ra.rev_shift = d.trace_len - 1;
+  }
   ra.n_events = d.n_events;
   ra.n_channels = d.n_channels;
   ra.norms = p->norms;
@@ -476,6 +594,59 @@ int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride,
   HIP_CHECK(launch_resolve(p->R, ra, 8192, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
+}
+
+static int check_execute_args(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                              float* vp, int* lp, float* vn, int* ln) {
+  if (!p || !d_traces || !vp || !lp || !vn || !ln) {
+    set_error("null argument");
+    return SXC_ERR_INVALID;
+  }
+  if (!p->lag && (in_len < 1 || in_len > p->P || trace_stride < in_len)) {
+    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  if (p->lag && (in_len > trace_stride || in_len < 1)) {
+    set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  return SXC_OK;
+}
+
+extern "C" {
+
+int sxc_plan_execute(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                     float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                     void* stream) {
+  clear_error();
+  int rc = check_execute_args(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg, d_lags_neg);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  p->last_stream = st;
+  if (p->lag)
+    return lag_plan_execute(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,
+                            d_lags_neg, st);
+  return fft_plan_execute(p, d_traces, nullptr, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,
+                          d_lags_neg, st);
+}
+
+int sxc_plan_execute_reversed(sxc_plan* p, const float* d_traces, const float* d_reversed,
+                              long long trace_stride, int in_len, float* d_vals_pos, int* d_lags_pos,
+                              float* d_vals_neg, int* d_lags_neg, void* stream) {
+  clear_error();
+  int rc = check_execute_args(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg, d_lags_neg);
+  if (rc) return rc;
+  if (!d_reversed) { set_error("null argument"); return SXC_ERR_INVALID; }
+  if (p->lag) { set_error("reversed rows are an FFT-form execute (the lag forms correlate the traces directly)"); return SXC_ERR_INVALID; }
+  if (p->d.demean) { set_error("reversed rows cannot be demeaned consistently: create the plan with demean = 0"); return SXC_ERR_INVALID; }
+  if (!p->col_spec &&
+      (rc = dev_alloc(&p->col_spec, (size_t)((long long)p->d.n_events * p->d.n_channels * p->spec_stride),
+                      "reversed-row spectra")))
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  p->last_stream = st;
+  return fft_plan_execute(p, d_traces, d_reversed, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,
+                          d_lags_neg, st);
 }
 
 int sxc_plan_enable_timing(sxc_plan* p, int on) {
@@ -529,57 +700,15 @@ int sxc_plan_stats(sxc_plan* p, sxc_stats* out) {
 int sxc_allpairs_host(const sxc_plan_desc* desc, const float* traces, long long trace_stride,
                       int in_len, float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg) {
   clear_error();
-  if (!desc || !traces || !vals_pos || !lags_pos || !vals_neg || !lags_neg) {
-    set_error("null argument");
-    return SXC_ERR_INVALID;
-  }
-  sxc_plan* p = nullptr;
-  int rc = sxc_plan_create(desc, &p);
-  if (rc) return rc;
-  const long long rows = (long long)desc->n_events * desc->n_channels;
-  const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
-  const size_t np = (size_t)p->n_pairs;
-  float *d_in = nullptr, *d_vp = nullptr, *d_vn = nullptr;
-  int *d_lp = nullptr, *d_ln = nullptr;
-  std::string err;
-  auto cleanup = [&]() {
-    void* bufs[] = {d_in, d_vp, d_vn, d_lp, d_ln};
-    for (void* b : bufs)
-      if (b) (void)hipFree(b);
-    sxc_plan_destroy(p);
-  };
-  if ((rc = dev_alloc(&d_in, (size_t)(rows * trace_stride), "input traces")) ||
-      (rc = dev_alloc(&d_vp, np, "output values")) || (rc = dev_alloc(&d_vn, np, "output values")) ||
-      (rc = dev_alloc(&d_lp, np, "output lags")) || (rc = dev_alloc(&d_ln, np, "output lags"))) {
-    cleanup();
-    return rc;
-  }
-  hipError_t e = hipMemcpy(d_in, traces, in_bytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    set_error("copying traces to the device failed: %s", hipGetErrorString(e));
-    cleanup();
-    return SXC_ERR_HIP;
-  }
-  rc = sxc_plan_execute(p, d_in, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr);
-  if (rc == SXC_OK) {
-    hipError_t es = hipDeviceSynchronize();
-    if (es != hipSuccess) {
-      set_error("device execution failed: %s", hipGetErrorString(es));
-      rc = SXC_ERR_HIP;
-    }
-  }
-  if (rc == SXC_OK) {
-    hipError_t e1 = hipMemcpy(vals_pos, d_vp, np * sizeof(float), hipMemcpyDeviceToHost);
-    hipError_t e2 = hipMemcpy(lags_pos, d_lp, np * sizeof(int), hipMemcpyDeviceToHost);
-    hipError_t e3 = hipMemcpy(vals_neg, d_vn, np * sizeof(float), hipMemcpyDeviceToHost);
-    hipError_t e4 = hipMemcpy(lags_neg, d_ln, np * sizeof(int), hipMemcpyDeviceToHost);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
-      set_error("copying results to the host failed");
-      rc = SXC_ERR_HIP;
-    }
-  }
-  cleanup();
-  return rc;
+  return allpairs_host(desc, traces, nullptr, trace_stride, in_len, vals_pos, lags_pos, vals_neg, lags_neg);
+}
+
+int sxc_allpairs_host_reversed(const sxc_plan_desc* desc, const float* traces, const float* reversed,
+                               long long trace_stride, int in_len, float* vals_pos, int* lags_pos,
+                               float* vals_neg, int* lags_neg) {
+  clear_error();
+  if (!reversed) { set_error("null argument"); return SXC_ERR_INVALID; }
+  return allpairs_host(desc, traces, reversed, trace_stride, in_len, vals_pos, lags_pos, vals_neg, lags_neg);
 }
 
 }  // extern "C"
@@ -611,10 +740,22 @@ int reference_desc(int n_events, int event_length, int shift, int padded, float 
   return SXC_OK;
 }
 
-void reference_call(float* events, int n_events, int event_length, int shift, int padded,
-                    float threshold, float* vp, int* lp, float* vn, int* ln) {
+// Lag window of the CUDA ABIs.  By default they scan lags -shift+1 .. shift
+// like the C/OMP path (correlation_c.cpp:102-111).  The CUDA references scan
+// only whole 128-thread strides, lags -128 k + 1 .. 128 k with k = floor(shift
+// / 128) (Correlation_in_CUDA/correlation_cuda.cu:114-117, NUM_THREADS :8;
+// Correlation_in_MultiCUDA/correlation_multicuda.cu FE_Linear); setting
+// SEISMIC_XCORR_WINDOW=cuda in the environment selects that window.
+int cuda_abi_shift(int shift) {
+  const char* w = getenv("SEISMIC_XCORR_WINDOW");
+  if (w && strcmp(w, "cuda") == 0 && shift > 0) return shift / 128 * 128;
+  return shift;
+}
+
+void reference_call(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                    int padded, float threshold, float* vp, int* lp, float* vn, int* ln) {
   clear_error();
-  if (!events || !vp || !lp || !vn || !ln) {
+  if (!events || !events_reversed || !vp || !lp || !vn || !ln) {
     set_error("null pointer argument");
     return;
   }
@@ -628,19 +769,20 @@ void reference_call(float* events, int n_events, int event_length, int shift, in
     memset(ln, 0, np * sizeof(int));
     return;
   }
-  // the reference FFTs the whole padded row and normalises by the first L samples
-  sxc_allpairs_host(&d, events, padded, padded, vp, lp, vn, ln);
+  // the reference FFTs the whole padded rows (events and reversed events) and
+  // normalises by the first L samples of the events
+  allpairs_host(&d, events, events_reversed, padded, padded, vp, lp, vn, ln);
 }
 
 // OMP ABI (Correlation_in_OMP/correlation_c.cpp:209-298): the same job, plus
 // the per-class histograms of every pair's pos and neg coefficient taken
 // before the threshold and added (+=) into the caller's arrays (:284-288).
 // The pairs run without a threshold; the histogram pass then applies it.
-void reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
-                        float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+void reference_call_omp(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                        int padded, float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
                         int* max_hist, int* min_hist) {
   clear_error();
-  if (!events || !vp || !lp || !vn || !ln || !max_hist || !min_hist) {
+  if (!events || !events_reversed || !vp || !lp || !vn || !ln || !max_hist || !min_hist) {
     set_error("null pointer argument");
     return;
   }
@@ -663,16 +805,17 @@ void reference_call_omp(float* events, int n_events, int event_length, int shift
   d.threshold = -INFINITY;
   sxc_plan* p = nullptr;
   if (sxc_plan_create(&d, &p)) return;
-  float *d_in = nullptr, *d_vp = nullptr, *d_vn = nullptr;
+  float *d_in = nullptr, *d_rev = nullptr, *d_vp = nullptr, *d_vn = nullptr;
   int *d_lp = nullptr, *d_ln = nullptr, *d_hist = nullptr;
   auto cleanup = [&]() {
-    void* bufs[] = {d_in, d_vp, d_vn, d_lp, d_ln, d_hist};
+    void* bufs[] = {d_in, d_rev, d_vp, d_vn, d_lp, d_ln, d_hist};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     sxc_plan_destroy(p);
   };
   const size_t n = (size_t)np;
-  if (dev_alloc(&d_in, (size_t)n_events * padded, "input traces") || dev_alloc(&d_vp, n, "output values") ||
+  if (dev_alloc(&d_in, (size_t)n_events * padded, "input traces") ||
+      dev_alloc(&d_rev, (size_t)n_events * padded, "reversed traces") || dev_alloc(&d_vp, n, "output values") ||
       dev_alloc(&d_vn, n, "output values") || dev_alloc(&d_lp, n, "output lags") ||
       dev_alloc(&d_ln, n, "output lags") || dev_alloc(&d_hist, 2 * (size_t)n_classes, "histograms")) {
     cleanup();
@@ -680,8 +823,12 @@ void reference_call_omp(float* events, int n_events, int event_length, int shift
   }
   std::vector<int> h(2 * (size_t)n_classes);
   bool ok = hipMemcpy(d_in, events, (size_t)n_events * padded * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(d_rev, events_reversed, (size_t)n_events * padded * sizeof(float), hipMemcpyHostToDevice) ==
+                hipSuccess &&
             hipMemset(d_hist, 0, 2 * (size_t)n_classes * sizeof(int)) == hipSuccess;
-  if (ok) ok = sxc_plan_execute(p, d_in, padded, padded, d_vp, d_lp, d_vn, d_ln, nullptr) == SXC_OK;
+  if (ok)
+    ok = sxc_plan_execute_reversed(p, d_in, d_rev, padded, padded, d_vp, d_lp, d_vn, d_ln, nullptr) == SXC_OK;
+  int tie_rc = SXC_OK;
   if (ok) {
     HistArgs ha{};
     ha.vpos = d_vp;
@@ -696,6 +843,7 @@ void reference_call_omp(float* events, int n_events, int event_length, int shift
     ha.hist_neg = d_hist + n_classes;
     ok = launch_hist_threshold(ha, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     if (!ok) set_error("histogram pass failed");
+    if (ok) tie_rc = check_tie_overflow(p);
   }
   if (ok)
     ok = hipMemcpy(vp, d_vp, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
@@ -711,6 +859,7 @@ void reference_call_omp(float* events, int n_events, int event_length, int shift
   } else if (!last_error()[0]) {
     set_error("device execution or copy failed");
   }
+  (void)tie_rc;  // an overflow has set the error string; the results are delivered
   cleanup();
 }
 
@@ -810,16 +959,17 @@ int sxc_device_synchronize(void) {
   return SXC_OK;
 }
 
-void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
-                        float threshold, float* vp, int* lp, float* vn, int* ln) {
-  sxc::reference_call(events, n_events, event_length, shift, padded, threshold, vp, lp, vn, ln);
+void sxc_reference_call(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                        int padded, float threshold, float* vp, int* lp, float* vn, int* ln) {
+  sxc::reference_call(events, events_reversed, n_events, event_length, sxc::cuda_abi_shift(shift), padded,
+                      threshold, vp, lp, vn, ln);
 }
 
-void sxc_reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
-                            float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+void sxc_reference_call_omp(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                            int padded, float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
                             int* max_hist, int* min_hist) {
-  sxc::reference_call_omp(events, n_events, event_length, shift, padded, threshold, n_classes, vp, lp, vn,
-                          ln, max_hist, min_hist);
+  sxc::reference_call_omp(events, events_reversed, n_events, event_length, shift, padded, threshold, n_classes,
+                          vp, lp, vn, ln, max_hist, min_hist);
 }
 int sxc_warm_up(void) { return sxc::warm_up(); }
 }

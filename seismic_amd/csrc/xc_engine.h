@@ -24,21 +24,25 @@ int validate_desc(const sxc_plan_desc& d);
 // Maps the reference ABI arguments to a plan description (lags -shift+1..shift).
 int reference_desc(int n_events, int event_length, int shift, int padded, float threshold,
                    sxc_plan_desc* d);
-// Full drop-in call on host buffers; errors are reported via last_error().
-void reference_call(float* events, int n_events, int event_length, int shift, int padded,
-                    float threshold, float* vp, int* lp, float* vn, int* ln);
-void reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
-                        float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+// Full drop-in call on host buffers (events and the caller's reversed events,
+// both [n_events][padded]); errors are reported via last_error().
+void reference_call(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                    int padded, float threshold, float* vp, int* lp, float* vn, int* ln);
+void reference_call_omp(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                        int padded, float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
                         int* max_hist, int* min_hist);
+// shift of the CUDA ABIs: as given, or floor(shift / 128) * 128 when
+// SEISMIC_XCORR_WINDOW=cuda selects the CUDA references' window
+int cuda_abi_shift(int shift);
 int warm_up();
 }  // namespace sxc
 
 extern "C" {
 // Exported for the drop-in shims (same library namespace).
-void sxc_reference_call(float* events, int n_events, int event_length, int shift, int padded,
-                        float threshold, float* vp, int* lp, float* vn, int* ln);
-void sxc_reference_call_omp(float* events, int n_events, int event_length, int shift, int padded,
-                            float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
+void sxc_reference_call(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                        int padded, float threshold, float* vp, int* lp, float* vn, int* ln);
+void sxc_reference_call_omp(float* events, float* events_reversed, int n_events, int event_length, int shift,
+                            int padded, float threshold, int n_classes, float* vp, int* lp, float* vn, int* ln,
                             int* max_hist, int* min_hist);
 int sxc_warm_up(void);
 }

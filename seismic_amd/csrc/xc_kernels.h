@@ -22,6 +22,16 @@ struct ForwardArgs {
   const float2* post_tw;  // exp(-2 pi i k / P), k = 0..M
   const int* row_len;     // optional per-event sample counts (rows of one event share it);
                           // when set it replaces in_len and L for that row
+  // Reversed-row mode (the reference's second R2C, Correlation_in_OMP/
+  // correlation_c.cpp:43-45): the rows are the caller's reversed traces r_j.
+  // The energy is not summed here; the scale comes from norms_in (the forward
+  // rows' sqrt(E)), and the row is stored as the column spectrum
+  // conj(rfft(r_j)[k] * exp(+2 pi i k rev_shift / P)) * scale, so that the
+  // pair kernel's Xhat_i conj(col) is X_i R_j rotated by rev_shift = L - 1:
+  // its lag-l slot then holds the reference's corr_t[L - 1 + l].
+  int reversed;
+  int rev_shift;
+  const double* norms_in;
 };
 
 struct PairArgs {
@@ -52,6 +62,12 @@ struct PairArgs {
 struct ResolveArgs {
   const float2* spec;
   long long spec_stride;
+  const float2* col_spec; // column spectra (null: spec); reversed-row mode when rev_traces is set
+  const float* rev_traces;// reversed-row mode: [n_events * n_channels][rev_stride] r_j, zero beyond rev_len;
+                          // the lag-l value is sum_m x_i[m] r_j[(l + rev_shift - m) mod P]
+  long long rev_stride;
+  int rev_len;
+  int rev_shift;
   int n_events;
   int n_channels;
   const double* norms;    // [n_events * n_channels]

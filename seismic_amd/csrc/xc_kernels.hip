@@ -128,20 +128,27 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   }
   if (a.in_len < 1) {  // empty row: zero spectrum
     for (int k = lane; k <= M; k += 64) a.spec[(long long)row * a.spec_stride + k] = make_float2(0.f, 0.f);
-    if (lane == 0) a.norms[row] = 0.0;
+    if (lane == 0 && !a.reversed) a.norms[row] = 0.0;
     return;
   }
-  // mean and energy of the first L samples, FP64, straight from memory
-  double s1 = 0.0;
-  for (int n = lane; n < a.L; n += 64) s1 += (double)(n < a.in_len ? x[n] : 0.f);
-  const double mean = a.demean ? wave_sum_f64(s1) / (double)a.L : 0.0;
-  double s2 = 0.0;
-  for (int n = lane; n < a.L; n += 64) {
-    const float v = n < a.in_len ? x[n] : 0.f;
-    const float u = a.demean ? (float)((double)v - mean) : v;
-    s2 += (double)u * (double)u;
+  // mean and energy of the first L samples, FP64, straight from memory (a
+  // reversed row takes the energy of its forward row instead)
+  double mean = 0.0, E = 0.0;
+  if (a.reversed) {
+    const double nr = a.norms_in[row];
+    E = nr * nr;
+  } else {
+    double s1 = 0.0;
+    for (int n = lane; n < a.L; n += 64) s1 += (double)(n < a.in_len ? x[n] : 0.f);
+    mean = a.demean ? wave_sum_f64(s1) / (double)a.L : 0.0;
+    double s2 = 0.0;
+    for (int n = lane; n < a.L; n += 64) {
+      const float v = n < a.in_len ? x[n] : 0.f;
+      const float u = a.demean ? (float)((double)v - mean) : v;
+      s2 += (double)u * (double)u;
+    }
+    E = wave_sum_f64(s2);
   }
-  const double E = wave_sum_f64(s2);
   // d[m] = x[2(t+64m)] + i x[2(t+64m)+1]; clamped loads + selects, no branches
   cf d[R];
   const int last = a.in_len > 0 ? a.in_len - 1 : 0;
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
       *reinterpret_cast<float2*>(o + n) = make_float2(d[m].x, d[m].y);
     }
   }
-  if (lane == 0) a.norms[row] = sqrt(E);
+  if (lane == 0 && !a.reversed) a.norms[row] = sqrt(E);
   const float scale = E > 0.0 ? (float)(1.0 / sqrt(E * (double)a.n_channels * (double)P)) : 0.f;
 
   wave_fft<R, -1>(d, twl, wl, lane);
@@ -188,6 +195,15 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
     cf od = cmk(0.5f * dd.y, -0.5f * dd.x);  // -i/2 * dd
     cf t = f2c(a.post_tw[k]);
     cf X = cadd(e, cmul(od, t));
+    if (a.reversed) {
+      // X * exp(+2 pi i q / P), q = k * rev_shift mod P, from the table of
+      // exp(-2 pi i k / P), k <= M: conj(post[q]) for q <= M, post[P - q] above
+      const int q = (int)(((long long)k * a.rev_shift) % P);
+      const cf pq = f2c(a.post_tw[q <= M ? q : P - q]);
+      const cf ph = cmk(pq.x, q <= M ? -pq.y : pq.y);
+      X = cmul(X, ph);
+      X.y = -X.y;  // stored conjugated: the pair kernel conjugates its column operand
+    }
     if (k == 0 || k == M) X.y = 0.f;
     out[k] = make_float2(X.x * scale, X.y * scale);
   }
@@ -512,6 +528,22 @@ __device__ double corr_f64(const float* __restrict__ ti, const float* __restrict
   return wave_sum_f64(s);
 }
 
+// The reference's own sum for reversed rows (Correlation_in_OMP/
+// correlation_c.cpp:270, :63, :102-103): its lag-l value is corr_t[L - 1 + l],
+// the circular convolution sum_m x_i[m] r_j[(q - m) mod P] at q = L - 1 + l.
+__device__ double conv_f64(const float* __restrict__ ti, int len_i, const float* __restrict__ rj, int len_j,
+                           int P, int q, int lane) {
+  q %= P;
+  q += q < 0 ? P : 0;
+  double s = 0.0;
+  for (int m = lane; m < len_i; m += 64) {
+    int u = q - m;  // q, m in [0, P): one wrap suffices
+    u += u < 0 ? P : 0;
+    if (u < len_j) s += (double)ti[m] * (double)rj[u];
+  }
+  return wave_sum_f64(s);
+}
+
 template <int R>
 __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   constexpr int M = 64 * R;
@@ -536,8 +568,9 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
     const int i = t.x, j = t.y, side = t.z;
     wave_lds_fence();
     cf y[R];
+    const float2* cols = a.col_spec ? a.col_spec : a.spec;
     pair_inverse<R, true, true>(y, a.spec + (long long)i * a.n_channels * a.spec_stride, (int)a.spec_stride,
-                          a.spec + (long long)j * a.n_channels * a.spec_stride, a.spec_stride,
+                          cols + (long long)j * a.n_channels * a.spec_stride, a.spec_stride,
                           a.n_channels, twl, wl, lane);
     apply_window<R>(y, a.winmask, lane);
     const long long idx = packed_index(i, j, a.n_events);
@@ -585,8 +618,15 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
             const double nj = a.norms[(long long)j * a.n_channels + c];
             if (!(ni > 0.0 && nj > 0.0)) continue;
             const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
-            const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
-            acc += corr_f64(ti, tj, a.tr_len, P, lag, lane) / (ni * nj * (double)a.n_channels);
+            double cij;
+            if (a.rev_traces) {
+              const float* rj = a.rev_traces + ((long long)j * a.n_channels + c) * a.rev_stride;
+              cij = conv_f64(ti, a.tr_len, rj, a.rev_len, P, lag + a.rev_shift, lane);
+            } else {
+              const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+              cij = corr_f64(ti, tj, a.tr_len, P, lag, lane);
+            }
+            acc += cij / (ni * nj * (double)a.n_channels);
           }
           acc *= (double)sg;
           if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {

@@ -5,6 +5,9 @@
 namespace sxc {
 
 constexpr int kLagBlock = 16;  // lags per block (one accumulator tile each)
+// The tile kernel keeps lags as 16-bit offsets from the first lag of block 0,
+// so a plan's window spans at most this many lags (whole blocks).
+constexpr int kLagMaxSpan = 65536;
 
 struct LagPrepArgs {
   const float* in;          // [n_rows][in_stride]
@@ -43,7 +46,8 @@ struct LagPairArgs {
   int* lpos;
   float* vneg;
   int* lneg;
-  int4* ties;               // 2 int4 per record: (i, j, sides, 0), (lp1, lp2, ln1, ln2)
+  int4* ties;               // 2 int4 per record: (i, j, sides, 0), (lp1, lp2, ln1, ln2); sides bit 0/1:
+                            // pos/neg near tie, bit 2/3: a third lag of that side is in the band too
   unsigned* tie_count;
   int tie_cap;
 };

@@ -208,11 +208,33 @@ __device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsign
 }
 
 // Running per-pair state of one wave, in LDS, lane-contiguous so every
-// access is bank-conflict free: [wave][pair r][pos1, pos2, neg1, neg2][lane]
+// access is bank-conflict free: [wave][pair r][...][lane].  Per side the three
+// largest values (pos: p1 >= p2 >= p3; neg: n1 <= n2 <= n3) and the lags of
+// the first two as 16-bit offsets from lag_first, packed into one word.
 struct LagState {
-  float v[4][16][4][64];
-  int l[4][16][4][64];
+  float v[4][16][6][64];  // p1 p2 p3 n1 n2 n3
+  unsigned l[4][16][2][64];  // (lp1 | lp2 << 16), (ln1 | ln2 << 16) - lag_first
 };
+
+// Merge of two descending top-3 lists (values a/b, lags of the first two) into
+// the top three of their union; ties of the first place go to the smaller lag
+// (the two lists cover disjoint lag blocks).  The negative side calls it with
+// negated values.
+__device__ __forceinline__ void merge_top3(float& a1, float& a2, float& a3, unsigned& la1, unsigned& la2,
+                                           float b1, float b2, float b3, unsigned lb1, unsigned lb2) {
+  const bool take_b = b1 > a1 || (b1 == a1 && lb1 < la1);
+  // W: the list whose head wins, O: the other
+  const float w1 = take_b ? b1 : a1, w2 = take_b ? b2 : a2, w3 = take_b ? b3 : a3;
+  const unsigned lw1 = take_b ? lb1 : la1, lw2 = take_b ? lb2 : la2;
+  const float o1 = take_b ? a1 : b1, o2 = take_b ? a2 : b2;
+  const unsigned lo1 = take_b ? la1 : lb1;
+  const bool o_second = o1 > w2;
+  a1 = w1;
+  la1 = lw1;
+  a2 = o_second ? o1 : w2;
+  la2 = o_second ? lo1 : lw2;
+  a3 = o_second ? fmaxf(w2, o2) : fmaxf(w3, o1);
+}
 
 template <bool BF16X3>
 __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
@@ -235,43 +257,46 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     __syncthreads();  // the previous tile's merge has read the state
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      st.v[wv][r][0][lane] = 0.f; st.v[wv][r][1][lane] = -3.0e38f;
-      st.v[wv][r][2][lane] = 0.f; st.v[wv][r][3][lane] = 3.0e38f;
-      st.l[wv][r][0][lane] = 0; st.l[wv][r][1][lane] = 0; st.l[wv][r][2][lane] = 0; st.l[wv][r][3][lane] = 0;
+      st.v[wv][r][0][lane] = 0.f; st.v[wv][r][1][lane] = -3.0e38f; st.v[wv][r][2][lane] = -3.0e38f;
+      st.v[wv][r][3][lane] = 0.f; st.v[wv][r][4][lane] = 3.0e38f; st.v[wv][r][5][lane] = 3.0e38f;
+      st.l[wv][r][0][lane] = 0u; st.l[wv][r][1][lane] = 0u;
     }
     for (int b = wv; b < a.n_blocks; b += 4) {
       const int lb = a.lag_first + kLagBlock * b;
       f32x16 acc[kLagBlock];
       lag_block<BF16X3>(acc, ahi, alo, bhi, blo, a.ls, a.n_channels, a.l16, a.p0, lb, h);
-      // top-2 per pair over the block's lags, in increasing lag order
+      // top-3 values (top-2 lags) per pair over the block's lags, in increasing lag order
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        float p1 = st.v[wv][r][0][lane], p2 = st.v[wv][r][1][lane];
-        float n1 = st.v[wv][r][2][lane], n2 = st.v[wv][r][3][lane];
-        int lp1 = st.l[wv][r][0][lane], lp2 = st.l[wv][r][1][lane];
-        int ln1 = st.l[wv][r][2][lane], ln2 = st.l[wv][r][3][lane];
+        float p1 = st.v[wv][r][0][lane], p2 = st.v[wv][r][1][lane], p3 = st.v[wv][r][2][lane];
+        float n1 = st.v[wv][r][3][lane], n2 = st.v[wv][r][4][lane], n3 = st.v[wv][r][5][lane];
+        const unsigned wp = st.l[wv][r][0][lane], wn = st.l[wv][r][1][lane];
+        unsigned lp1 = wp & 0xffffu, lp2 = wp >> 16, ln1 = wn & 0xffffu, ln2 = wn >> 16;
 #pragma unroll
         for (int d = 0; d < kLagBlock; ++d) {
           const int lag = lb + d;
+          const unsigned off = (unsigned)(kLagBlock * b + d);
           const bool in = lag >= a.lag_lo && lag <= a.lag_hi;
           const float v = in ? acc[d][r] : 0.f;
           const bool gp = v > p1;
           const bool sp = !gp && v > p2;
+          p3 = (gp || sp) ? p2 : fmaxf(p3, v);
           p2 = gp ? p1 : (sp ? v : p2);
-          lp2 = gp ? lp1 : (sp ? lag : lp2);
+          lp2 = gp ? lp1 : (sp ? off : lp2);
           p1 = gp ? v : p1;
-          lp1 = gp ? lag : lp1;
+          lp1 = gp ? off : lp1;
           const bool gn = v < n1;
           const bool sn = !gn && v < n2;
+          n3 = (gn || sn) ? n2 : fminf(n3, v);
           n2 = gn ? n1 : (sn ? v : n2);
-          ln2 = gn ? ln1 : (sn ? lag : ln2);
+          ln2 = gn ? ln1 : (sn ? off : ln2);
           n1 = gn ? v : n1;
-          ln1 = gn ? lag : ln1;
+          ln1 = gn ? off : ln1;
         }
-        st.v[wv][r][0][lane] = p1; st.v[wv][r][1][lane] = p2;
-        st.v[wv][r][2][lane] = n1; st.v[wv][r][3][lane] = n2;
-        st.l[wv][r][0][lane] = lp1; st.l[wv][r][1][lane] = lp2;
-        st.l[wv][r][2][lane] = ln1; st.l[wv][r][3][lane] = ln2;
+        st.v[wv][r][0][lane] = p1; st.v[wv][r][1][lane] = p2; st.v[wv][r][2][lane] = p3;
+        st.v[wv][r][3][lane] = n1; st.v[wv][r][4][lane] = n2; st.v[wv][r][5][lane] = n3;
+        st.l[wv][r][0][lane] = lp1 | (lp2 << 16);
+        st.l[wv][r][1][lane] = ln1 | (ln2 << 16);
       }
     }
     // merge the 4 waves' states (lag blocks interleave, so ties go to the smaller lag)
@@ -279,35 +304,18 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     if (wv == 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        float p1 = st.v[0][r][0][lane], p2 = st.v[0][r][1][lane];
-        float n1 = st.v[0][r][2][lane], n2 = st.v[0][r][3][lane];
-        int lp1 = st.l[0][r][0][lane], lp2 = st.l[0][r][1][lane];
-        int ln1 = st.l[0][r][2][lane], ln2 = st.l[0][r][3][lane];
+        float p1 = st.v[0][r][0][lane], p2 = st.v[0][r][1][lane], p3 = st.v[0][r][2][lane];
+        float n1 = -st.v[0][r][3][lane], n2 = -st.v[0][r][4][lane], n3 = -st.v[0][r][5][lane];
+        unsigned lp1 = st.l[0][r][0][lane] & 0xffffu, lp2 = st.l[0][r][0][lane] >> 16;
+        unsigned ln1 = st.l[0][r][1][lane] & 0xffffu, ln2 = st.l[0][r][1][lane] >> 16;
         for (int w = 1; w < 4; ++w) {
-          const float q1 = st.v[w][r][0][lane], q2 = st.v[w][r][1][lane];
-          const int lq1 = st.l[w][r][0][lane], lq2 = st.l[w][r][1][lane];
-          // positive side; the starting 0 at lag 0 loses every tie
-          const bool takeq = q1 > p1 || (q1 == p1 && q1 > 0.f && lq1 < lp1);
-          float s1, s2;
-          int ls1, ls2;
-          if (takeq) { s1 = q1; ls1 = lq1; s2 = p1; ls2 = lp1; }
-          else { s1 = p1; ls1 = lp1; s2 = q1; ls2 = lq1; }
-          const float o2 = fmaxf(p2, q2);
-          const int lo2 = p2 >= q2 ? lp2 : lq2;
-          if (o2 > s2) { s2 = o2; ls2 = lo2; }
-          p1 = s1; lp1 = ls1; p2 = s2; lp2 = ls2;
-          const float m1 = st.v[w][r][2][lane], m2 = st.v[w][r][3][lane];
-          const int lm1 = st.l[w][r][2][lane], lm2 = st.l[w][r][3][lane];
-          const bool takem = m1 < n1 || (m1 == n1 && m1 < 0.f && lm1 < ln1);
-          float t1, t2;
-          int lt1, lt2;
-          if (takem) { t1 = m1; lt1 = lm1; t2 = n1; lt2 = ln1; }
-          else { t1 = n1; lt1 = ln1; t2 = m1; lt2 = lm1; }
-          const float u2 = fminf(n2, m2);
-          const int lu2 = n2 <= m2 ? ln2 : lm2;
-          if (u2 < t2) { t2 = u2; lt2 = lu2; }
-          n1 = t1; ln1 = lt1; n2 = t2; ln2 = lt2;
+          const unsigned qp = st.l[w][r][0][lane], qn = st.l[w][r][1][lane];
+          merge_top3(p1, p2, p3, lp1, lp2, st.v[w][r][0][lane], st.v[w][r][1][lane], st.v[w][r][2][lane],
+                     qp & 0xffffu, qp >> 16);
+          merge_top3(n1, n2, n3, ln1, ln2, -st.v[w][r][3][lane], -st.v[w][r][4][lane], -st.v[w][r][5][lane],
+                     qn & 0xffffu, qn >> 16);
         }
+        n1 = -n1; n2 = -n2; n3 = -n3;
         const int i = i0 + mfma_row(lane, r);
         const int j = j0 + (lane & 31);
         if (i < a.n_events && j < a.n_events && j >= i) {
@@ -315,17 +323,21 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
                                          : (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
           const bool kp = p1 > a.threshold && p1 > 0.f;
           const bool kn = n1 < -a.threshold && n1 < 0.f;
+          const int glp1 = a.lag_first + (int)lp1, gln1 = a.lag_first + (int)ln1;
           a.vpos[idx] = kp ? p1 : 0.f;
-          a.lpos[idx] = kp ? lp1 : 0;
+          a.lpos[idx] = kp ? glp1 : 0;
           a.vneg[idx] = kn ? n1 : 0.f;
-          a.lneg[idx] = kn ? ln1 : 0;
+          a.lneg[idx] = kn ? gln1 : 0;
           const bool tp = kp && (p1 - p2 < a.tie_tol || p1 < a.tie_tol);
           const bool tn = kn && (n2 - n1 < a.tie_tol || -n1 < a.tie_tol);
+          // a third lag within the band: every in-window lag is a candidate
+          const bool wp = tp && (p1 - p3 < a.tie_tol || p1 < a.tie_tol);
+          const bool wn = tn && (n3 - n1 < a.tie_tol || -n1 < a.tie_tol);
           if (tp || tn) {
             const unsigned slot = atomicAdd(a.tie_count, 1u);
             if (slot < (unsigned)a.tie_cap) {
-              a.ties[2 * slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0), 0);
-              a.ties[2 * slot + 1] = make_int4(lp1, lp2, ln1, ln2);
+              a.ties[2 * slot] = make_int4(i, j, (tp ? 1 : 0) | (tn ? 2 : 0) | (wp ? 4 : 0) | (wn ? 8 : 0), 0);
+              a.ties[2 * slot + 1] = make_int4(glp1, a.lag_first + (int)lp2, gln1, a.lag_first + (int)ln2);
             }
           }
         }
@@ -360,6 +372,56 @@ __device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, con
   return s;
 }
 
+// Every lag of [lag_lo, lag_hi] at once, one lag per lane: FP64 sum of
+// sg * c_ij(l) over channels (already normalised), and the wave's best by the
+// reference scan rule (largest above 0, smallest lag on exact ties).  For the
+// records whose third-best FP32 value is also inside the band.
+__device__ __forceinline__ void sweep_window_f64(const LagResolveArgs& a, int i, int j, double sg, int lane,
+                                                 double& best, int& best_lag, bool& have) {
+  for (int l0 = a.lag_lo; l0 <= a.lag_hi; l0 += 64) {
+    const int lag = l0 + lane;
+    double acc = 0.0;
+    for (int c = 0; c < a.n_channels; ++c) {
+      const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
+      if (!(ni > 0.0 && nj > 0.0)) continue;
+      const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+      const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+      // sum_n ti[n] tj[n - lag] over 0 <= n < L, 0 <= n - lag < L
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      auto term = [&](int n) -> double {
+        const int u = n - lag;
+        const bool ok = u >= 0 && u < a.L;
+        const float b = tj[ok ? u : 0];
+        return ok ? (double)ti[n] * (double)b : 0.0;
+      };
+      int n = 0;
+      for (; n + 3 < a.L; n += 4) {
+        s0 += term(n);
+        s1 += term(n + 1);
+        s2 += term(n + 2);
+        s3 += term(n + 3);
+      }
+      for (; n < a.L; ++n) s0 += term(n);
+      acc += ((s0 + s1) + (s2 + s3)) / (ni * nj * (double)a.n_channels);
+    }
+    acc *= sg;
+    // wave argmax of this pass: larger value, then smaller lag
+    double v = (lag <= a.lag_hi && acc > 0.0) ? acc : -1.0;
+    int lg = lag;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(v, o, 64);
+      const int ol = __shfl_xor(lg, o, 64);
+      if (ov > v || (ov == v && ol < lg)) { v = ov; lg = ol; }
+    }
+    if (v > 0.0 && (!have || v > best)) {  // passes run in increasing lag order
+      best = v;
+      best_lag = lg;
+      have = true;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
   const int lane = threadIdx.x;
   const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
@@ -372,32 +434,38 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
   const unsigned rec0 = xcd * seg + local, seg_end = min(total, (xcd + 1) * seg);
   for (unsigned rec = rec0; rec < seg_end; rec += n_local) {
     const int4 t0 = a.ties[2 * rec], t1 = a.ties[2 * rec + 1];
-    const int i = t0.x, j = t0.y, sides = t0.z;
+    const int i = __builtin_amdgcn_readfirstlane(t0.x), j = __builtin_amdgcn_readfirstlane(t0.y);
+    const int sides = __builtin_amdgcn_readfirstlane(t0.z);
     const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
                                    : (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
     for (int sd = 0; sd < 2; ++sd) {
       if (!(sides & (1 << sd))) continue;
-      const int c0 = sd == 0 ? t1.x : t1.z, c1 = sd == 0 ? t1.y : t1.w;
       const double sg = sd == 0 ? 1.0 : -1.0;
       double best = 0.0;
       int best_lag = 0;
       bool have = false;
-      for (int q = 0; q < 2; ++q) {
-        const int lag = q == 0 ? c0 : c1;
-        if (lag < a.lag_lo || lag > a.lag_hi || (q == 1 && c1 == c0)) continue;
-        double acc = 0.0;
-        for (int c = 0; c < a.n_channels; ++c) {
-          const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
-          if (!(ni > 0.0 && nj > 0.0)) continue;
-          const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
-          const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
-          acc += lin_corr_f64(ti, tj, a.L, lag, lane) / (ni * nj * (double)a.n_channels);
-        }
-        acc *= sg;
-        if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
-          best = acc;
-          best_lag = lag;
-          have = true;
+      if (sides & (4 << sd)) {
+        sweep_window_f64(a, i, j, sg, lane, best, best_lag, have);
+      } else {
+        // only the two recorded lags can hold the FP64 winner
+        const int c0 = sd == 0 ? t1.x : t1.z, c1 = sd == 0 ? t1.y : t1.w;
+        for (int q = 0; q < 2; ++q) {
+          const int lag = q == 0 ? c0 : c1;
+          if (lag < a.lag_lo || lag > a.lag_hi || (q == 1 && c1 == c0)) continue;
+          double acc = 0.0;
+          for (int c = 0; c < a.n_channels; ++c) {
+            const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
+            if (!(ni > 0.0 && nj > 0.0)) continue;
+            const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+            const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+            acc += lin_corr_f64(ti, tj, a.L, lag, lane) / (ni * nj * (double)a.n_channels);
+          }
+          acc *= sg;
+          if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
+            best = acc;
+            best_lag = lag;
+            have = true;
+          }
         }
       }
       if (lane == 0) {

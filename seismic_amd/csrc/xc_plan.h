@@ -41,6 +41,7 @@ struct sxc_plan {
   float tie_tol = 1e-5f;
   // device workspace
   float2* spec = nullptr;
+  float2* col_spec = nullptr;  // reversed-row executes: column spectra (allocated on first use)
   double* norms = nullptr;
   float* traces = nullptr;     // demeaned copy for the FP64 resolver (demean only)
   int2* work = nullptr;
@@ -63,6 +64,16 @@ struct sxc_plan {
 };
 
 void plan_free(sxc_plan* p);
+namespace sxc {
+// FFT form: fills the plan's geometry (no device memory).
+void fft_plan_geometry(const sxc_plan_desc& desc, long long tie_cap_max, sxc_plan* p);
+// FFT form: launches forward (and reversed-row) R2C, the pair kernel and the
+// FP64 resolver on the plan's workspace (asynchronous on st).
+int fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride, int in_len,
+                     float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg, hipStream_t st);
+// lane masks of the in-window slots of the inverse wave FFT of R values per lane
+std::vector<unsigned long long> window_masks(int R, int lag_lo, int lag_hi);
+}  // namespace sxc
 // lag-window forms: planes, norms, FP32 traces, triangle tile list, tie records
 int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out);
 // lag-window forms: FP64 energy, scaled bf16 hi/lo (or fp16) planes and the

@@ -16,6 +16,17 @@
 // slot count is bounded by the device-memory budget (memory-based
 // admission); a request larger than a slot is rejected at submit.
 //
+// Deadline admission (sxc_server_set_latency_budget).  Each request is priced
+// in algorithmic FFT-form flops: its events x the catalog x
+// (C * 8 * (P/2 + 1) + 2.5 P log2 P) at each event's period P, plus a fixed
+// per-request host/launch cost.  A fluid clock, busy_until, is when the device
+// would finish everything admitted so far at the modelled rate (calibrated by
+// timing two batches at create and refreshed from the device's measured busy
+// time).  A request whose predicted completion lies more than the budget after
+// its arrival is rejected at submit with SXC_ERR_OVERLOAD -- unless the device
+// is idle, so a request larger than the budget is still served when nothing
+// else waits.  The FIFO is also bounded (kMaxQueued requests).
+//
 // Lags.  Query q (length Lq) against catalog event c (length Lc) uses the FFT
 // period P: the smallest of 1024, 2048, 3072, 4096, 5120, 6144, 8192 with
 // P >= Lq + Lc - 1, so the period-P correlation equals the linear one on every
@@ -59,6 +70,11 @@ using Clock = std::chrono::steady_clock;
   } while (0)
 
 constexpr int kPeriods[] = {1024, 2048, 3072, 4096, 5120, 6144, 8192};
+constexpr size_t kMaxQueued = 4096;  // requests waiting for a slot
+
+// Algorithmic FFT-form flops of one query event against one catalog event at
+// period P: C conj-multiply-accumulates per bin and one stacked C2R.
+double pair_flops(int P, int C) { return (double)C * 8.0 * (P / 2 + 1) + 2.5 * P * log2((double)P); }
 
 // Smallest FFT period whose circular correlation equals the linear one on
 // every overlapping lag -(lc-1) .. lq-1 (P >= lq + lc - 1).
@@ -109,6 +125,7 @@ void tables_free(Tables* t) {
 struct Request {
   long long id = 0;
   int n = 0;
+  double flops = 0.0;         // algorithmic work (admission model)
   std::vector<float> events;  // [n][C][max_query_len]
   std::vector<int> lengths;
   std::vector<sxc_match> result;
@@ -167,8 +184,18 @@ struct sxc_server {
   // statistics
   std::vector<double> latencies_ms;
   unsigned long long events_done = 0, batches = 0, requests_done = 0, batched_requests = 0;
+  unsigned long long offered = 0, rejected = 0;
   Clock::time_point t_first_submit, t_last_done;
   bool any_submit = false;
+  // deadline admission (budget_ms <= 0: off)
+  Clock::time_point t0 = Clock::now();  // origin of the ms clocks below
+  double budget_ms = 0.0;
+  double rate_flops_per_ms = 0.0;       // modelled device rate
+  double overhead_ms = 0.0;             // modelled fixed cost per request
+  double busy_until_ms = 0.0;           // fluid clock of the admitted work
+  int in_flight = 0;                    // batches on the device
+  Clock::time_point busy_start;
+  double busy_ms_acc = 0.0, flops_done_acc = 0.0;  // measured since the last rate update
 };
 
 namespace {
@@ -316,6 +343,7 @@ void worker_loop(sxc_server* S, int slot_index) {
         batch.push_back(S->queue.front());
         S->queue.pop_front();
       }
+      if (S->in_flight++ == 0) S->busy_start = Clock::now();
     }
     // pack events grouped by period: (request, event) -> row
     std::map<int, std::vector<std::pair<int, int>>> by_period;  // P -> (request idx, event idx)
@@ -372,9 +400,62 @@ void worker_loop(sxc_server* S, int slot_index) {
       S->batches += 1;
       S->batched_requests += batch.size();
       S->t_last_done = now;
+      for (auto& q : batch) S->flops_done_acc += q->flops;
+      if (--S->in_flight == 0) {
+        S->busy_ms_acc += std::chrono::duration<double, std::milli>(now - S->busy_start).count();
+        const double now_ms = std::chrono::duration<double, std::milli>(now - S->t0).count();
+        // every admitted request has finished: the fluid clock cannot be ahead of now
+        if (S->queue.empty()) S->busy_until_ms = std::min(S->busy_until_ms, now_ms);
+        // refresh the modelled rate from the device's measured busy time
+        // (whole busy periods, at least 200 ms of them, weighted 1:3 with the old rate)
+        if (S->busy_ms_acc >= 200.0 && S->rate_flops_per_ms > 0.0) {
+          const double measured = S->flops_done_acc / S->busy_ms_acc;
+          S->rate_flops_per_ms = 0.75 * S->rate_flops_per_ms + 0.25 * measured;
+          S->busy_ms_acc = S->flops_done_acc = 0.0;
+        }
+      }
     }
     S->cv_done.notify_all();
   }
+}
+
+// Times two batches of catalog events used as queries (1 event, then 64) at
+// the period of the longest accepted query, on slot 0 before the workers
+// start: rate = the extra events' flops over the extra time, overhead = the
+// 1-event batch's time beyond its flops at that rate.
+int calibrate(sxc_server* S, const float* catalog) {
+  const sxc_server_desc& d = S->d;
+  Slot* s = &S->slots[0];
+  const int C = d.n_channels, Lq = d.max_query_len;
+  const int len = std::min(d.max_query_len, d.catalog_len);
+  const int P = period_for(len, d.catalog_len);
+  const int big = std::min(64, d.max_batch_events);
+  for (int e = 0; e < big; ++e) {
+    const int src = e % d.n_catalog;
+    for (int c = 0; c < C; ++c) {
+      float* dst = s->h_traces + ((size_t)e * C + c) * Lq;
+      memset(dst, 0, (size_t)Lq * sizeof(float));
+      memcpy(dst, catalog + ((size_t)src * C + c) * d.catalog_len, (size_t)len * sizeof(float));
+    }
+    s->h_len[e] = len;
+  }
+  auto timed = [&](int n, double* ms) {
+    std::vector<std::pair<int, int>> g{{0, n}};
+    int rc = run_batch(S, s, g);  // warm-up (first launches of each kernel)
+    if (rc) return rc;
+    const auto a = Clock::now();
+    rc = run_batch(S, s, g);
+    *ms = std::chrono::duration<double, std::milli>(Clock::now() - a).count();
+    return rc;
+  };
+  double t1 = 0.0, tb = 0.0;
+  int rc = timed(1, &t1);
+  if (!rc && big > 1) rc = timed(big, &tb);
+  if (rc) return rc;
+  const double f = pair_flops(P, C) * d.n_catalog;  // per event
+  S->rate_flops_per_ms = (big > 1 && tb > t1) ? f * (big - 1) / (tb - t1) : f / std::max(t1, 1e-3);
+  S->overhead_ms = std::max(0.0, t1 - f / S->rate_flops_per_ms);
+  return SXC_OK;
 }
 
 double percentile(std::vector<double> v, double p) {
@@ -487,8 +568,21 @@ int sxc_server_create(const sxc_server_desc* desc, const float* catalog, sxc_ser
     int rc = slot_create(S, &S->slots[i]);
     if (rc) return fail(rc);
   }
+  {
+    int rc = calibrate(S, catalog);
+    if (rc) return fail(rc);
+  }
   for (int i = 0; i < n_slots; ++i) S->workers.emplace_back(worker_loop, S, i);
   *out = S;
+  return SXC_OK;
+}
+
+int sxc_server_set_latency_budget(sxc_server* S, double budget_ms) {
+  clear_error();
+  if (!S) { set_error("null server"); return SXC_ERR_INVALID; }
+  if (!(budget_ms >= 0.0)) { set_error("budget_ms must be >= 0 (got %g)", budget_ms); return SXC_ERR_INVALID; }
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->budget_ms = budget_ms;
   return SXC_OK;
 }
 
@@ -517,9 +611,31 @@ int sxc_server_submit(sxc_server* S, const float* events, long long stride, cons
     for (int c = 0; c < C; ++c)
       memcpy(q->events.data() + ((size_t)e * C + c) * Lq, events + ((size_t)e * C + c) * stride,
              (size_t)q->lengths[e] * sizeof(float));
+  for (int e = 0; e < n_events; ++e)
+    q->flops += pair_flops(period_for(q->lengths[e], d.catalog_len), C) * d.n_catalog;
   q->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(S->mu);
+    S->offered += 1;
+    if (S->queue.size() >= kMaxQueued) {
+      S->rejected += 1;
+      set_error("request rejected: %zu requests already wait for a slot", S->queue.size());
+      return SXC_ERR_OVERLOAD;
+    }
+    const double now_ms = std::chrono::duration<double, std::milli>(q->t_submit - S->t0).count();
+    const double start_ms = std::max(now_ms, S->busy_until_ms);
+    const double done_ms = start_ms + q->flops / S->rate_flops_per_ms;
+    if (S->budget_ms > 0.0) {
+      const double predicted = done_ms + S->overhead_ms - now_ms;
+      const bool idle = S->queue.empty() && S->in_flight == 0;
+      if (predicted > S->budget_ms && !idle) {
+        S->rejected += 1;
+        set_error("request rejected: predicted latency %.1f ms exceeds the %.1f ms budget", predicted,
+                  S->budget_ms);
+        return SXC_ERR_OVERLOAD;
+      }
+    }
+    S->busy_until_ms = done_ms;
     q->id = S->next_id++;
     if (!S->any_submit) { S->t_first_submit = q->t_submit; S->any_submit = true; }
     S->requests[q->id] = q;
@@ -568,6 +684,11 @@ int sxc_server_stats(sxc_server* S, sxc_serve_stats* out) {
   const double span = S->any_submit ? std::chrono::duration<double>(S->t_last_done - S->t_first_submit).count() : 0.0;
   out->events_per_s = span > 0 ? (double)S->events_done / span : 0.0;
   out->slot_bytes = slot_bytes(S->d, S->spec_stride_max, S->n_chunks);
+  out->offered = S->offered;
+  out->rejected = S->rejected;
+  out->budget_ms = S->budget_ms;
+  out->model_tflops = S->rate_flops_per_ms / 1e9;
+  out->model_overhead_ms = S->overhead_ms;
   return SXC_OK;
 }
 
@@ -577,6 +698,7 @@ int sxc_server_reset_stats(sxc_server* S) {
   std::lock_guard<std::mutex> lk(S->mu);
   S->latencies_ms.clear();
   S->events_done = S->batches = S->requests_done = S->batched_requests = 0;
+  S->offered = S->rejected = 0;
   S->any_submit = false;
   return SXC_OK;
 }

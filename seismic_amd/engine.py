@@ -53,29 +53,48 @@ class AllPairsPlan:
         return (DeviceArray(n, np.float32), DeviceArray(n, np.int32),
                 DeviceArray(n, np.float32), DeviceArray(n, np.int32))
 
-    def execute(self, traces, in_len=None, out=None, stream=None):
-        """traces: DeviceArray float32 [N, C, stride] (or [N, stride] for C=1).
-        Asynchronous on `stream` (a device.Stream; None = default stream)."""
+    def _check_io(self, traces, out):
         if traces.dtype != np.float32:
             raise ValueError("traces must be float32")
         stride = traces.shape[-1]
         if int(np.prod(traces.shape)) != self.n_events * self.n_channels * stride:
             raise ValueError("traces shape does not match the plan")
-        in_len = stride if in_len is None else int(in_len)
         if out is None:
             out = self.alloc_outputs()
-        vp, lp, vn, ln = out
         # the kernels write n_pairs entries into each output: a smaller
         # buffer would be written out of bounds on the device
         for o, dt in zip(out, (np.float32, np.int32, np.float32, np.int32)):
             if o.dtype != dt or o.nbytes < self.n_pairs * 4:
                 raise ValueError(f"outputs must be {self.n_pairs}-entry float32/int32/float32/int32 "
                                  f"device arrays (got {o.dtype} of {o.nbytes} bytes)")
+        return stride, out
+
+    def execute(self, traces, in_len=None, out=None, stream=None):
+        """traces: DeviceArray float32 [N, C, stride] (or [N, stride] for C=1).
+        Asynchronous on `stream` (a device.Stream; None = default stream)."""
+        stride, out = self._check_io(traces, out)
+        in_len = stride if in_len is None else int(in_len)
+        vp, lp, vn, ln = out
         sh = stream.handle if stream is not None else 0
         rc = self.lib.sxc_plan_execute(self.handle, traces.ptr, stride, in_len,
                                        vp.ptr, lp.ptr, vn.ptr, ln.ptr, ctypes.c_void_p(sh))
         if rc != 0:
             raise RuntimeError(f"sxc_plan_execute failed ({rc}): {last_error(self.lib)}")
+        return out
+
+    def execute_reversed(self, traces, reversed_traces, in_len=None, out=None, stream=None):
+        """The reference's X_i * R_j with the caller's reversed rows
+        (sxc_plan_execute_reversed); both DeviceArrays of the same shape."""
+        stride, out = self._check_io(traces, out)
+        if reversed_traces.dtype != np.float32 or reversed_traces.shape != traces.shape:
+            raise ValueError("reversed traces must be float32 of the traces' shape")
+        in_len = stride if in_len is None else int(in_len)
+        vp, lp, vn, ln = out
+        sh = stream.handle if stream is not None else 0
+        rc = self.lib.sxc_plan_execute_reversed(self.handle, traces.ptr, reversed_traces.ptr, stride, in_len,
+                                                vp.ptr, lp.ptr, vn.ptr, ln.ptr, ctypes.c_void_p(sh))
+        if rc != 0:
+            raise RuntimeError(f"sxc_plan_execute_reversed failed ({rc}): {last_error(self.lib)}")
         return out
 
     def stats(self):

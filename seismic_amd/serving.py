@@ -13,11 +13,21 @@ from ._lib import (SxcMatch, SxcServerDesc, SxcServerStats, c_float_p, c_int_p, 
                    load_engine)
 
 
+SXC_ERR_OVERLOAD = -5
+
+
+class Overloaded(RuntimeError):
+    """A submit shed by the server's admission (latency budget or queue bound);
+    the request was not queued and may be submitted again later."""
+
+
 class Server:
     def __init__(self, catalog: np.ndarray, max_query_len: int, top_k: int = 10,
                  max_batch_events: int = 256, n_slots: int = 4, mem_budget: int = 0,
-                 demean: bool = False):
-        """catalog: [n_catalog, n_channels, catalog_len] float32."""
+                 demean: bool = False, latency_budget_ms: float = 0.0):
+        """catalog: [n_catalog, n_channels, catalog_len] float32.
+        latency_budget_ms > 0 turns on deadline admission (see
+        set_latency_budget)."""
         self.lib = load_engine()
         cat = np.ascontiguousarray(catalog, dtype=np.float32)
         n, c, L = cat.shape
@@ -32,6 +42,15 @@ class Server:
             raise RuntimeError(f"sxc_server_create failed ({rc}): {last_error(self.lib)}")
         self.handle = h
         self._pending = {}
+        if latency_budget_ms:
+            self.set_latency_budget(latency_budget_ms)
+
+    def set_latency_budget(self, budget_ms: float) -> None:
+        """Submits whose predicted latency exceeds budget_ms raise Overloaded
+        (unless the device is idle); 0 turns admission by deadline off."""
+        rc = self.lib.sxc_server_set_latency_budget(self.handle, float(budget_ms))
+        if rc != 0:
+            raise RuntimeError(f"sxc_server_set_latency_budget failed ({rc}): {last_error(self.lib)}")
 
     def submit(self, events: np.ndarray, lengths) -> int:
         """events: [n, n_channels, stride] float32 ([n, stride] with one
@@ -47,6 +66,8 @@ class Server:
         rid = ctypes.c_longlong()
         rc = self.lib.sxc_server_submit(self.handle, ev.ctypes.data_as(c_float_p), ev.shape[-1],
                                         ln.ctypes.data_as(c_int_p), ev.shape[0], ctypes.byref(rid))
+        if rc == SXC_ERR_OVERLOAD:
+            raise Overloaded(last_error(self.lib))
         if rc != 0:
             raise RuntimeError(f"sxc_server_submit failed ({rc}): {last_error(self.lib)}")
         self._pending[rid.value] = ev.shape[0]
@@ -107,9 +128,11 @@ def make_requests(n_requests, n_channels, len_range, size_range, seed, pool_even
     return reqs
 
 
-def run_poisson(server: Server, requests, rate_hz: float, seed: int):
-    """Submits `requests` at Poisson arrival times (rate_hz) from one thread,
-    waits for all, returns per-request latencies and timing."""
+def run_poisson(server: Server, requests, rate_hz: float, seed: int, collect=False):
+    """Submits `requests` at Poisson arrival times (rate_hz) from one thread
+    and waits for the admitted ones.  Returns per-request latencies (NaN for
+    a request shed by admission), the admitted mask, sizes and timing;
+    collect=True also returns every admitted request's results."""
     rng = np.random.Generator(np.random.PCG64(seed))
     gaps = rng.exponential(1.0 / rate_hz, size=len(requests))
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -121,16 +144,29 @@ def run_poisson(server: Server, requests, rate_hz: float, seed: int):
             delay = t0 + arrivals[k] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            ids[k] = server.submit(ev, lens)
+            try:
+                ids[k] = server.submit(ev, lens)
+            except Overloaded:
+                ids[k] = -1
 
     th = threading.Thread(target=submitter)
     th.start()
-    lat = np.zeros(len(requests))
+    lat = np.full(len(requests), np.nan)
     sizes = np.array([r[0].shape[0] for r in requests])
+    results = [None] * len(requests)
     for k in range(len(requests)):
         while ids[k] is None:
             time.sleep(0.0005)
-        lat[k] = server.wait(ids[k])[3]
+        if ids[k] < 0:
+            continue
+        r = server.wait(ids[k])
+        lat[k] = r[3]
+        if collect:
+            results[k] = r[:3]
     th.join()
     wall = time.perf_counter() - t0
-    return {"latency_ms": lat, "sizes": sizes, "wall_s": wall, "offered_s": float(arrivals[-1])}
+    out = {"latency_ms": lat, "admitted": ~np.isnan(lat), "sizes": sizes, "wall_s": wall,
+           "offered_s": float(arrivals[-1])}
+    if collect:
+        out["results"] = results
+    return out

@@ -6,7 +6,7 @@ Bar (stated per test):
   lags: equal to the oracle's C/OMP restatement except where the FP64 gap
         between the top two values is below TIE_TOL, and equal to the FP64
         argmax wherever that gap is at least GPU_EXACT_GAP (the GPU settles
-        near ties in FP64 itself).
+        near ties in FP64 itself, so only exact ties are exempt).
 """
 import numpy as np
 import pytest
@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 COEF_TOL = 1e-5
 TIE_TOL = 1e-5
-GPU_EXACT_GAP = 2e-6
+GPU_EXACT_GAP = 1e-12
 
 
 def _run_dropin(ev2d, shift, threshold=0.0):
@@ -99,24 +99,30 @@ def test_known_answers_gpu():
     assert not vp2.any() and not lp2.any() and not vn2.any() and not ln2.any()
 
 
-def test_exact_ties_take_first_lag():
-    """Autocorrelations are symmetric (exact ties at +-l for the minimum) and a
-    period-P/4 cosine has equal maxima every P/4 samples: the FP64 tie-break
-    must return the smallest lag, as the reference scan's first occurrence."""
+def test_exact_ties_take_first_lag(oracle):
+    """The autocorrelation of integer-valued noise is exactly symmetric in
+    FP64 (every partial sum of the oracle's direct sums is an exact integer),
+    so its minimum at -l and +l is an exact tie, and a period-64 cosine has
+    near-equal maxima every 64 samples: the FP64 tie-break must return the
+    smaller lag on the exact tie, as the reference scan's first occurrence,
+    and the FP64 argmax otherwise."""
     L = 1024
-    P = 2048
     t = np.arange(L)
     cosine = np.cos(2 * np.pi * t / 64.0).astype(np.float32)
     rng = np.random.default_rng(9)
-    noise = rng.standard_normal(L).astype(np.float32)
+    noise = rng.integers(-8, 9, size=L).astype(np.float32)
     ev = np.stack([cosine, noise])
     vp, lp, vn, ln = _run_dropin(ev, 1024, 0.0)
+    pi, pj = oracle.packed_pairs(2)
+    exact = oracle.pairs_f64(ev, pi, pj, -1023, 1024, P=2048, circular=True)
+    k = 2  # pair (1, 1)
+    assert exact["gap_neg"][k] == 0.0, "integer autocorrelation: exact FP64 tie at -l and +l"
+    assert exact["lag_neg"][k] < 0
+    assert ln[k] == exact["lag_neg"][k]
+    assert lp[0] == 0 and abs(vp[0] - 1) < 1e-5
     padded, rev, L_, P_ = synth.pad_like_binding(ev)
     ref = np_ref.reference_f64(padded, rev, L_, 1024)
-    # autocorrelation of noise: min is attained at +-l, first is the negative lag
-    k = 2  # pair (1, 1): the minimum sits at +-l; either sign is an FP64 tie
-    assert abs(ln[k]) == abs(ref["lagn"][k])
-    assert lp[0] == 0 and abs(vp[0] - 1) < 1e-5
+    _check((vp, lp, vn, ln), ref, None, "cosine and integer noise")
 
 
 def test_multicuda_matches_cuda():
@@ -309,6 +315,8 @@ def test_dropin_padded_3072(oracle):
     ev, _ = synth.make_events(n, 1, L, seed=3072)
     rows = np.zeros((n, P), np.float32)
     rows[:, :L] = ev[:, 0]
+    rev = np.zeros((n, P), np.float32)
+    rev[:, :L] = ev[:, 0, ::-1]  # the binding's reversed rows (correlation_lib.py:61-62)
     m = n * (n + 1) // 2
     vp, vn = np.zeros(m, np.float32), np.zeros(m, np.float32)
     lp, ln = np.zeros(m, np.int32), np.zeros(m, np.int32)
@@ -316,7 +324,7 @@ def test_dropin_padded_3072(oracle):
     fp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
     print("case drop-in N=24 L=1500 paddedSize=3072", flush=True)
-    lib.correlationCUDA(fp(rows), fp(rows), n, L, shift, P, 1, 0.0, fp(vp), ip(lp), fp(vn), ip(ln))
+    lib.correlationCUDA(fp(rows), fp(rev), n, L, shift, P, 1, 0.0, fp(vp), ip(lp), fp(vn), ip(ln))
     assert lib.seismic_last_error() == b""
     pi, pj = oracle.packed_pairs(n)
     ref = oracle.pairs_f64(ev, pi, pj, -shift + 1, shift, P=P, circular=True)

@@ -295,3 +295,65 @@ def test_serving_one_channel_demean_top32_length_extremes():
         assert np.abs(v[e] - rv).max() <= 1e-5, e
         if e == 0:   # a one-sample query has zero energy after mean removal: every match is 0
             assert np.all(v[e] == 0)
+
+
+def test_latency_budget_argument_checks():
+    """The admission setter refuses a null server and a negative budget
+    without touching a device."""
+    import ctypes
+    from seismic_amd import _lib
+    lib = _lib.load_engine()
+    assert lib.sxc_server_set_latency_budget(None, ctypes.c_double(10.0)) == -1
+    assert b"null" in lib.seismic_last_error()
+
+
+@pytest.mark.gpu
+def test_deadline_admission_sheds_and_keeps_results():
+    """A burst submitted at once against a small latency budget: the server
+    sheds what its work model predicts would finish late (Overloaded, not
+    queued), serves the rest, and every admitted request's results are
+    bitwise those of the same request served alone.  A request larger than
+    the budget is still admitted onto an idle device, and budget 0 admits
+    everything."""
+    cat, _ = synth.make_events(2048, 3, 1024, seed=61)
+    srv = serving.Server(cat, max_query_len=2000, top_k=5, max_batch_events=64, n_slots=3)
+    st0 = srv.stats()
+    print(f"model: {st0['model_tflops']:.1f} TFLOP/s, {st0['model_overhead_ms']:.2f} ms per request")
+    assert st0["model_tflops"] > 0
+    reqs = serving.make_requests(40, 3, (300, 2000), (16, 64), seed=62, pool_events=64)
+    alone = [srv.wait(srv.submit(e, ln))[:3] for e, ln in reqs]
+    srv.reset_stats()
+    srv.set_latency_budget(4.0)
+    ids = []
+    for e, ln in reqs:
+        try:
+            ids.append(srv.submit(e, ln))
+        except serving.Overloaded as ex:
+            assert "budget" in str(ex)
+            ids.append(None)
+    got = [srv.wait(i)[:3] if i is not None else None for i in ids]
+    st = srv.stats()
+    shed = sum(i is None for i in ids)
+    print(f"burst of 40: {shed} shed, {40 - shed} served; stats offered {st['offered']} rejected {st['rejected']}")
+    assert st["offered"] == 40 and st["rejected"] == shed and st["requests"] == 40 - shed
+    assert 0 < shed < 40
+    for k, g in enumerate(got):
+        if g is not None:
+            for a, b in zip(alone[k], g):
+                assert np.array_equal(a, b)
+    # idle device: a request predicted above the budget is still served
+    srv.set_latency_budget(0.01)
+    big = reqs[0]
+    r = srv.wait(srv.submit(*big))
+    for a, b in zip(alone[0], r[:3]):
+        assert np.array_equal(a, b)
+    # admission off: the whole burst is queued
+    srv.set_latency_budget(0.0)
+    srv.reset_stats()
+    ids = [srv.submit(e, ln) for e, ln in reqs]
+    for i in ids:
+        srv.wait(i)
+    assert srv.stats()["rejected"] == 0 and srv.stats()["requests"] == 40
+    with pytest.raises(RuntimeError, match="budget_ms"):
+        srv.set_latency_budget(-1.0)
+    srv.close()
