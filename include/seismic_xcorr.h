@@ -104,6 +104,24 @@ int sxc_plan_execute_reversed(sxc_plan* plan, const float* d_traces, const float
                               long long trace_stride, int in_len, float* d_vals_pos, int* d_lags_pos,
                               float* d_vals_neg, int* d_lags_neg, void* stream);
 
+/* sxc_plan_execute without a plan and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  FFT form only (demean allowed); the caller passes the
+ * device traces and the four packed output arrays as for sxc_plan_execute
+ * and gets the same results bit for bit (same kernels, same launch shapes).
+ * The workspace -- spectra, norms, work list, window masks, twiddle tables,
+ * near-tie records (at most 2^20 per call; beyond that a pair keeps its FP32
+ * lag) -- lives in one static device array of 128 MiB per GPU, held until
+ * the process ends; a job that needs more is refused with SXC_ERR_INVALID
+ * before any launch (about N * C * (P/2 + 32) * 8 bytes of spectra, e.g.
+ * N = 2048 x 3 channels at P = 4096).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (several streams or host threads)
+ * race, so callers must order them.  Asynchronous on `stream`. */
+int sxc_plan_execute_noalloc(const sxc_plan_desc* desc, const float* d_traces, long long trace_stride,
+                             int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg,
+                             int* d_lags_neg, void* stream);
+
 /* Counters of the last execute on this plan (synchronises its stream). */
 int sxc_plan_stats(sxc_plan* plan, sxc_stats* out);
 

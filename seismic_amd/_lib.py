@@ -93,7 +93,8 @@ class SxcBlockInfo(ctypes.Structure):
 # every symbol include/*.h declares, per library
 EXPORTS = {
     "libseismic_xcorr.so": ["seismic_last_error", "sxc_plan_create", "sxc_plan_destroy",
-                            "sxc_plan_execute", "sxc_plan_execute_reversed", "sxc_plan_stats",
+                            "sxc_plan_execute", "sxc_plan_execute_reversed", "sxc_plan_execute_noalloc",
+                            "sxc_plan_stats",
                             "sxc_allpairs_host", "sxc_allpairs_host_reversed",
                             "sxc_plan_enable_timing", "sxc_plan_timing", "sxc_device_count",
                             "sxc_device_alloc", "sxc_device_free", "sxc_memcpy_h2d",
@@ -136,6 +137,9 @@ def load_engine() -> ctypes.CDLL:
     lib.sxc_plan_destroy.argtypes = [vp]
     lib.sxc_plan_execute.restype = ctypes.c_int
     lib.sxc_plan_execute.argtypes = [vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, vp, vp]
+    lib.sxc_plan_execute_noalloc.restype = ctypes.c_int
+    lib.sxc_plan_execute_noalloc.argtypes = [ctypes.POINTER(SxcPlanDesc), vp, ctypes.c_longlong, ctypes.c_int,
+                                             vp, vp, vp, vp, vp]
     lib.sxc_plan_execute_reversed.restype = ctypes.c_int
     lib.sxc_plan_execute_reversed.argtypes = [vp, vp, vp, ctypes.c_longlong, ctypes.c_int, vp, vp, vp, vp,
                                               vp]

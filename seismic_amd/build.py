@@ -68,11 +68,12 @@ def build(force: bool = False, verbose: bool = True, jobs: int | None = None) ->
         kobjs.append(out)
         if force or _stale(out, [src] + hdrs):
             steps.append([HIPCC] + HIP_FLAGS + [f"-DSXC_R={r}", "-c", src, "-o", out])
-    src = os.path.join(CSRC, "xc_lag.hip")
-    out = os.path.join(OBJ, "xc_lag.o")
-    kobjs.append(out)
-    if force or _stale(out, [src] + hdrs):
-        steps.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", out])
+    for name in ("xc_lag.hip", "xc_noalloc.hip"):
+        src = os.path.join(CSRC, name)
+        out = os.path.join(OBJ, name.replace(".hip", ".o"))
+        kobjs.append(out)
+        if force or _stale(out, [src] + hdrs):
+            steps.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", out])
     hobjs = []
     for name in ("xc_engine.cpp", "xc_dispatch.cpp", "xc_serve.cpp", "xc_tiled.cpp"):
         src = os.path.join(CSRC, name)
