@@ -269,12 +269,23 @@ def serving_leg(n_catalog: int, stream_requests: int, fixed_requests: int, fixed
                "model_tflops": round(st["model_tflops"], 2)}
         if small.any() and small.sum() != ok.sum():
             res["p99_ms_requests_le_64_events"] = round(float(np.percentile(r["latency_ms"][small], 99)), 2)
+            by = {}
+            for lo, hi in ((1, 64), (65, 128), (129, 192), (193, 256)):
+                m = (r["sizes"] >= lo) & (r["sizes"] <= hi)
+                lat_b = r["latency_ms"][m & ok]
+                by[f"{lo}-{hi}"] = {"offered": int(m.sum()), "admitted": int((m & ok).sum()),
+                                    "p50_ms": round(float(np.percentile(lat_b, 50)), 2) if lat_b.size else None,
+                                    "p99_ms": round(float(np.percentile(lat_b, 99)), 2) if lat_b.size else None}
+            res["by_request_events"] = by
         return res
 
     if stream_requests > 0:
         reqs = serving.make_requests(stream_requests, 3, (512, 4096), (1, 256), seed=11)
         log(f"bench: serving stream, {stream_requests} requests at 200 req/s, budget {budget_ms} ms")
         out["stream_1_256_events"] = leg(reqs, 200.0, 12, budget_ms)
+        if budget_ms > 30.0:
+            log(f"bench: serving stream, {stream_requests} requests at 200 req/s, budget 30 ms")
+            out["stream_1_256_events_budget_30ms"] = leg(reqs, 200.0, 12, 30.0)
     if alone_requests > 0:
         reqs = serving.make_requests(alone_requests, 3, (512, 4096), (64, 64), seed=13)
         log(f"bench: serving 64-event requests one at a time, {alone_requests}")

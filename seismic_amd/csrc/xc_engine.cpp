@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -118,7 +119,8 @@ void plan_free(sxc_plan* p) {
   if (!p) return;
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->col_spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
-                  p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles};
+                  p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles,
+                  p->fnorms, p->wide, p->wide_count};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -256,7 +258,74 @@ int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out) {
   }
   if ((rc = dev_alloc(&p->ties, (size_t)p->tie_cap * 2, "near-tie records"))) return fail(rc);
   if ((rc = dev_alloc(&p->tie_count, 1, "near-tie counter"))) return fail(rc);
+  // FFT period for the wide near ties: the period-P circular correlation of
+  // traces zero beyond L equals the linear one on every |lag| <= P - L
+  const int w = std::max(std::abs(desc->lag_lo), std::abs(desc->lag_hi));
+  for (int P : {1024, 2048, 3072, 4096, 5120, 6144, 8192})
+    if (P >= L + w) { p->P = P; break; }
+  if (p->P) {
+    p->M = p->P / 2;
+    p->R = p->M / 64;
+    p->spec_stride = p->M + 32;
+    if ((rc = dev_alloc(&p->spec, (size_t)(rows * p->spec_stride), "spectra"))) return fail(rc);
+    if ((rc = dev_alloc(&p->fnorms, (size_t)rows, "norms"))) return fail(rc);
+    if ((rc = dev_alloc(&p->wide, (size_t)p->tie_cap, "near-tie records"))) return fail(rc);
+    if ((rc = dev_alloc(&p->wide_count, 1, "near-tie counter"))) return fail(rc);
+    if ((rc = upload_tables(p))) return fail(rc);
+  }
   *out = p;
+  return SXC_OK;
+}
+
+int sxc::lag_plan_spectra(sxc_plan* p, hipStream_t st) {
+  if (!p->P) return SXC_OK;
+  const sxc_plan_desc& d = p->d;
+  ForwardArgs fa{};
+  fa.in = p->lag_traces;
+  fa.in_stride = d.trace_len;
+  fa.in_len = d.trace_len;
+  fa.L = d.trace_len;
+  fa.n_rows = d.n_events * d.n_channels;
+  fa.n_channels = d.n_channels;
+  fa.norms = p->fnorms;
+  fa.spec = p->spec;
+  fa.spec_stride = p->spec_stride;
+  fa.tw = p->fwd2;
+  fa.post_tw = p->post;
+  HIP_CHECK(launch_forward(p->R, fa, st));
+  return SXC_OK;
+}
+
+int sxc::lag_plan_resolve_wide(sxc_plan* p, float* vp, int* lp, float* vn, int* ln, long long out_ld, int out_row0,
+                               int out_col0, hipStream_t st) {
+  if (!p->P) return SXC_OK;
+  const sxc_plan_desc& d = p->d;
+  ResolveArgs ra{};
+  ra.spec = p->spec;
+  ra.spec_stride = p->spec_stride;
+  ra.n_events = d.n_events;
+  ra.n_channels = d.n_channels;
+  ra.norms = p->fnorms;
+  ra.traces = p->lag_traces;
+  ra.tr_stride = d.trace_len;
+  ra.tr_len = d.trace_len;
+  ra.lag_lo = d.lag_lo;
+  ra.lag_hi = d.lag_hi;
+  ra.threshold = d.threshold;
+  ra.tie_tol = 2e-6f;  // the FFT form's band: its FP32 values are within 5e-7 of FP64
+  ra.tw = p->inv2;
+  ra.winmask = p->winmask;
+  ra.vpos = vp;
+  ra.lpos = lp;
+  ra.vneg = vn;
+  ra.lneg = ln;
+  ra.ties = p->wide;
+  ra.tie_count = p->wide_count;
+  ra.tie_cap = p->tie_cap;
+  ra.out_ld = out_ld;
+  ra.out_row0 = out_row0;
+  ra.out_col0 = out_col0;
+  HIP_CHECK(launch_resolve(p->R, ra, 8192, st));
   return SXC_OK;
 }
 
@@ -290,7 +359,9 @@ static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_
   if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
   int rc = lag_plan_prep(p, d_traces, trace_stride, in_len, st);
   if (rc) return rc;
+  if ((rc = lag_plan_spectra(p, st))) return rc;
   HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
+  if (p->wide_count) HIP_CHECK(launch_fill_u32(p->wide_count, 0u, 1, st));
   if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
   LagPairArgs ka{};
   ka.hi = p->hi;
@@ -334,7 +405,11 @@ static int lag_plan_execute(sxc_plan* p, const float* d_traces, long long trace_
   ra.lpos = lp;
   ra.vneg = vn;
   ra.lneg = ln;
+  ra.wide = p->wide;
+  ra.wide_count = p->wide_count;
+  ra.wide_cap = p->tie_cap;
   HIP_CHECK(launch_lag_resolve(ra, 8192, st));
+  if ((rc = lag_plan_resolve_wide(p, vp, lp, vn, ln, 0, 0, 0, st))) return rc;
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
 }

@@ -86,6 +86,8 @@ struct ResolveArgs {
   const int4* ties;
   const unsigned* tie_count;
   int tie_cap;
+  long long out_ld;       // 0: packed triangle output; else dense block rows of this stride
+  int out_row0, out_col0; // event index of the block's first row / column
 };
 
 // Per-R launchers, one translation unit per R (xc_kernels.hip with -DSXC_R).

@@ -573,7 +573,8 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
                           cols + (long long)j * a.n_channels * a.spec_stride, a.spec_stride,
                           a.n_channels, twl, wl, lane);
     apply_window<R>(y, a.winmask, lane);
-    const long long idx = packed_index(i, j, a.n_events);
+    const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
+                                   : packed_index(i, j, a.n_events);
     for (int sd = 0; sd < 2; ++sd) {
       if (!(side & (1 << sd))) continue;
       const float sg = sd == 0 ? 1.f : -1.f;

@@ -68,6 +68,13 @@ struct LagResolveArgs {
   int* lneg;
   long long out_ld;         // as in LagPairArgs
   int out_row0, out_col0;
+  // Sides whose band held a third lag ("wide") are appended to this list as
+  // (i, j, sides, 0) for the FFT form's resolver when set; swept over the
+  // whole window in FP64 here otherwise.  One entry per processed record at
+  // most, so a list of tie_cap entries cannot overflow.
+  int4* wide;
+  unsigned* wide_count;
+  int wide_cap;
 };
 
 // FP32 value / int32 lag block -> float16 coefficient / int16 lag block

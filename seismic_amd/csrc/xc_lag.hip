@@ -375,7 +375,8 @@ __device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, con
 // Every lag of [lag_lo, lag_hi] at once, one lag per lane: FP64 sum of
 // sg * c_ij(l) over channels (already normalised), and the wave's best by the
 // reference scan rule (largest above 0, smallest lag on exact ties).  For the
-// records whose third-best FP32 value is also inside the band.
+// records whose third-best FP32 value is also inside the band, when the plan
+// has no FFT period to hand them to (no supported P >= L + max |lag|).
 __device__ __forceinline__ void sweep_window_f64(const LagResolveArgs& a, int i, int j, double sg, int lane,
                                                  double& best, int& best_lag, bool& have) {
   for (int l0 = a.lag_lo; l0 <= a.lag_hi; l0 += 64) {
@@ -438,6 +439,13 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
     const int sides = __builtin_amdgcn_readfirstlane(t0.z);
     const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
                                    : (long long)i * a.n_events - (long long)i * (i - 1) / 2 + (j - i);
+    // wide sides: any in-window lag may hold the FP64 winner; the FFT form's
+    // resolver finds the candidates from the pair's FP32 transform
+    const int wide_sides = (sides >> 2) & sides & 3;
+    if (a.wide && wide_sides && lane == 0) {
+      const unsigned slot = atomicAdd(a.wide_count, 1u);
+      if (slot < (unsigned)a.wide_cap) a.wide[slot] = make_int4(i, j, wide_sides, 0);
+    }
     for (int sd = 0; sd < 2; ++sd) {
       if (!(sides & (1 << sd))) continue;
       const double sg = sd == 0 ? 1.0 : -1.0;
@@ -445,6 +453,7 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
       int best_lag = 0;
       bool have = false;
       if (sides & (4 << sd)) {
+        if (a.wide) continue;  // handed to the FFT form's resolver
         sweep_window_f64(a, i, j, sg, lane, best, best_lag, have);
       } else {
         // only the two recorded lags can hold the FP64 winner

@@ -57,6 +57,12 @@ struct sxc_plan {
   unsigned short *hi = nullptr, *lo = nullptr;
   float* lag_traces = nullptr;
   int2* tiles = nullptr;
+  // lag-window forms: near ties with more than two lags in the band go to the
+  // FFT form's resolver at period P (0: no supported P >= L + max|lag|, they
+  // are swept in FP64 instead); spec holds the traces' spectra at P
+  double* fnorms = nullptr;
+  int4* wide = nullptr;
+  unsigned* wide_count = nullptr;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -76,6 +82,14 @@ std::vector<unsigned long long> window_masks(int R, int lag_lo, int lag_hi);
 }  // namespace sxc
 // lag-window forms: planes, norms, FP32 traces, triangle tile list, tie records
 int lag_plan_create(const sxc_plan_desc* desc, sxc_plan** out);
+namespace sxc {
+// lag-window forms: spectra of the FP32 traces at the plan's period P (when it has one)
+int lag_plan_spectra(sxc_plan* p, hipStream_t st);
+// lag-window forms: FP64 resolution of the wide near ties (the FFT form's resolver);
+// out_ld != 0 writes a dense block as the tiled runs do
+int lag_plan_resolve_wide(sxc_plan* p, float* vp, int* lp, float* vn, int* ln, long long out_ld, int out_row0,
+                          int out_col0, hipStream_t st);
+}  // namespace sxc
 // lag-window forms: FP64 energy, scaled bf16 hi/lo (or fp16) planes and the
 // FP32 copy of every trace
 int lag_plan_prep(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len, hipStream_t st);

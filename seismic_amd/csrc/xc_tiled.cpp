@@ -99,6 +99,7 @@ int enqueue_block(sxc_tiled* t, int b) {
   block_shape(t, b, &row0, &col0, &rows, &cols);
   hipStream_t st = t->s_comp;
   HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
+  if (p->wide_count) HIP_CHECK(launch_fill_u32(p->wide_count, 0u, 1, st));
   LagPairArgs ka{};
   ka.hi = p->hi;
   ka.lo = p->lo;
@@ -149,7 +150,12 @@ int enqueue_block(sxc_tiled* t, int b) {
   ra.out_ld = t->block;
   ra.out_row0 = row0;
   ra.out_col0 = col0;
+  ra.wide = p->wide;
+  ra.wide_count = p->wide_count;
+  ra.wide_cap = p->tie_cap;
   HIP_CHECK(launch_lag_resolve(ra, 8192, st));
+  int rc = lag_plan_resolve_wide(p, t->d_vp, t->d_lp, t->d_vn, t->d_ln, t->block, row0, col0, st);
+  if (rc) return rc;
   HIP_CHECK(hipMemcpyAsync(t->h_ties + slot, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   // slot reuse: block b - 2's copy out of this slot must be complete
   if (b >= 2) HIP_CHECK(hipStreamWaitEvent(st, t->e_copied[slot], 0));
@@ -272,6 +278,7 @@ int sxc_tiled_start(sxc_tiled* t, const float* d_traces, long long trace_stride,
   t->pairs = t->near_ties = t->tie_overflow = 0;
   int rc = lag_plan_prep(t->plan, d_traces, trace_stride, in_len, t->s_comp);
   if (rc) return rc;
+  if ((rc = lag_plan_spectra(t->plan, t->s_comp))) return rc;
   // the traces may be released by the caller once the prep has run
   HIP_CHECK(hipStreamSynchronize(t->s_comp));
   const int n = (int)t->blocks.size();
