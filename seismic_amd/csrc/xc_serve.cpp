@@ -22,8 +22,11 @@
 // per-request host/launch cost.  A fluid clock, busy_until, is when the device
 // would finish everything admitted so far at the modelled rate (calibrated by
 // timing two batches at create and refreshed from the device's measured busy
-// time).  A request whose predicted completion lies more than the budget after
-// its arrival is rejected at submit with SXC_ERR_OVERLOAD -- unless the device
+// time); each finished batch moves the clock by how much later or earlier it
+// finished than predicted at its admission, so model error does not
+// accumulate into a backlog.  A request whose predicted completion lies more
+// than the budget after its arrival is rejected at submit with
+// SXC_ERR_OVERLOAD -- unless the device
 // is idle, so a request larger than the budget is still served when nothing
 // else waits.  The FIFO is also bounded (kMaxQueued requests).
 //
@@ -126,6 +129,7 @@ struct Request {
   long long id = 0;
   int n = 0;
   double flops = 0.0;         // algorithmic work (admission model)
+  double pred_done_ms = 0.0;  // modelled completion at admission (server ms clock)
   std::vector<float> events;  // [n][C][max_query_len]
   std::vector<int> lengths;
   std::vector<sxc_match> result;
@@ -193,8 +197,8 @@ struct sxc_server {
   double rate_flops_per_ms = 0.0;       // modelled device rate
   double overhead_ms = 0.0;             // modelled fixed cost per request
   double busy_until_ms = 0.0;           // fluid clock of the admitted work
-  int in_flight = 0;                    // batches on the device
-  Clock::time_point busy_start;
+  int in_flight = 0;                    // batches taken from the queue, not yet finished
+  Clock::time_point busy_mark;          // last accounting instant while batches were in flight
   double busy_ms_acc = 0.0, flops_done_acc = 0.0;  // measured since the last rate update
 };
 
@@ -343,7 +347,7 @@ void worker_loop(sxc_server* S, int slot_index) {
         batch.push_back(S->queue.front());
         S->queue.pop_front();
       }
-      if (S->in_flight++ == 0) S->busy_start = Clock::now();
+      if (S->in_flight++ == 0) S->busy_mark = Clock::now();
     }
     // pack events grouped by period: (request, event) -> row
     std::map<int, std::vector<std::pair<int, int>>> by_period;  // P -> (request idx, event idx)
@@ -400,19 +404,30 @@ void worker_loop(sxc_server* S, int slot_index) {
       S->batches += 1;
       S->batched_requests += batch.size();
       S->t_last_done = now;
-      for (auto& q : batch) S->flops_done_acc += q->flops;
-      if (--S->in_flight == 0) {
-        S->busy_ms_acc += std::chrono::duration<double, std::milli>(now - S->busy_start).count();
-        const double now_ms = std::chrono::duration<double, std::milli>(now - S->t0).count();
+      double pred_ms = 0.0;
+      for (auto& q : batch) {
+        S->flops_done_acc += q->flops;
+        pred_ms = std::max(pred_ms, q->pred_done_ms);
+      }
+      S->busy_ms_acc += std::chrono::duration<double, std::milli>(now - S->busy_mark).count();
+      S->busy_mark = now;
+      const double now_ms = std::chrono::duration<double, std::milli>(now - S->t0).count();
+      if (--S->in_flight == 0 && S->queue.empty()) {
         // every admitted request has finished: the fluid clock cannot be ahead of now
-        if (S->queue.empty()) S->busy_until_ms = std::min(S->busy_until_ms, now_ms);
-        // refresh the modelled rate from the device's measured busy time
-        // (whole busy periods, at least 200 ms of them, weighted 1:3 with the old rate)
-        if (S->busy_ms_acc >= 200.0 && S->rate_flops_per_ms > 0.0) {
-          const double measured = S->flops_done_acc / S->busy_ms_acc;
-          S->rate_flops_per_ms = 0.75 * S->rate_flops_per_ms + 0.25 * measured;
-          S->busy_ms_acc = S->flops_done_acc = 0.0;
-        }
+        S->busy_until_ms = std::min(S->busy_until_ms, now_ms);
+      } else {
+        // closed loop: the batch finished later (earlier) than the clock
+        // predicted at its admission, and so will everything admitted after
+        // it -- shift the clock by the difference, so a rate the model has
+        // slightly wrong cannot build a backlog beyond the budget
+        S->busy_until_ms = std::max(now_ms, S->busy_until_ms + (now_ms - pred_ms));
+      }
+      // refresh the modelled rate from the device's measured busy time
+      // (at least 200 ms of it, weighted 1:3 with the old rate)
+      if (S->busy_ms_acc >= 200.0 && S->rate_flops_per_ms > 0.0) {
+        const double measured = S->flops_done_acc / S->busy_ms_acc;
+        S->rate_flops_per_ms = 0.75 * S->rate_flops_per_ms + 0.25 * measured;
+        S->busy_ms_acc = S->flops_done_acc = 0.0;
       }
     }
     S->cv_done.notify_all();
@@ -636,6 +651,7 @@ int sxc_server_submit(sxc_server* S, const float* events, long long stride, cons
       }
     }
     S->busy_until_ms = done_ms;
+    q->pred_done_ms = done_ms + S->overhead_ms;
     q->id = S->next_id++;
     if (!S->any_submit) { S->t_first_submit = q->t_submit; S->any_submit = true; }
     S->requests[q->id] = q;
