@@ -344,30 +344,35 @@ struct Best {
   int lag;
 };
 
-// Wave-wide max / min with DPP row operations (max and min are idempotent,
-// so lanes without a DPP source keep their own value); the result is read
-// from lane 63 and is wave-uniform.
-__device__ __forceinline__ float dpp_f(float v, int ctrl_sel) {
-  const int iv = __float_as_int(v);
-  int r = iv;
+// Wave-wide max / min with DPP row operations, read from lane 63 (so the
+// result is wave-uniform).  Both run on the float bit patterns as integers,
+// which DPP folds into the max itself (a float max would first quiet its DPP
+// operand: three more instructions per step): the running max is >= +0 (it
+// starts at 0), and non-negative floats order as signed integers (-0 maps to
+// the smallest, equal in value to +0); the running min is <= 0, and
+// non-positive floats order (reversed) as unsigned integers, with +0 below
+// -0 below every negative.  Lanes whose DPP source is out of range read 0,
+// which neither changes.
+__device__ __forceinline__ int dpp_i(int v, int ctrl_sel) {
   switch (ctrl_sel) {
-    case 0: r = __builtin_amdgcn_update_dpp(iv, iv, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]
-    case 1: r = __builtin_amdgcn_update_dpp(iv, iv, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
-    case 2: r = __builtin_amdgcn_update_dpp(iv, iv, 0x141, 0xF, 0xF, false); break;  // row_half_mirror
-    case 3: r = __builtin_amdgcn_update_dpp(iv, iv, 0x140, 0xF, 0xF, false); break;  // row_mirror
-    case 4: r = __builtin_amdgcn_update_dpp(iv, iv, 0x142, 0xF, 0xF, false); break;  // row_bcast:15
-    default: r = __builtin_amdgcn_update_dpp(iv, iv, 0x143, 0xF, 0xF, false); break; // row_bcast:31
+    case 0: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+    case 1: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+    case 2: return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    case 3: return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, true);  // row_mirror
+    case 4: return __builtin_amdgcn_update_dpp(0, v, 0x142, 0xF, 0xF, true);  // row_bcast:15
+    default: return __builtin_amdgcn_update_dpp(0, v, 0x143, 0xF, 0xF, true); // row_bcast:31
   }
-  return __int_as_float(r);
 }
 __device__ __forceinline__ void wave_maxmin(float& mx, float& mn) {
+  int imx = __float_as_int(mx);
+  unsigned umn = __float_as_uint(mn);
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    mx = fmaxf(mx, dpp_f(mx, k));
-    mn = fminf(mn, dpp_f(mn, k));
+    imx = max(imx, dpp_i(imx, k));
+    umn = max(umn, (unsigned)dpp_i((int)umn, k));
   }
-  mx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), 63));
-  mn = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mn), 63));
+  mx = __int_as_float(__builtin_amdgcn_readlane(imx, 63));
+  mn = __int_as_float(__builtin_amdgcn_readlane((int)umn, 63));
 }
 
 // Reference scan rule over the wave's values: strict > (<) starting from
