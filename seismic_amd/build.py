@@ -36,7 +36,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-D__HIP_PLATFORM_AMD__",
               f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
 KERNEL_RADICES = (8, 16, 24, 32, 40, 48, 64)
-HEADERS = ("xc_fft.hpp", "xc_kernels.h", "xc_engine.h", "xc_lag.h", "xc_plan.h")
+HEADERS = ("xc_fft.hpp", "xc_kernels.h", "xc_engine.h", "xc_lag.h", "xc_plan.h", "xc_hostio.h")
 
 
 def _run(cmd):
@@ -75,7 +75,7 @@ def build(force: bool = False, verbose: bool = True, jobs: int | None = None) ->
         if force or _stale(out, [src] + hdrs):
             steps.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", out])
     hobjs = []
-    for name in ("xc_engine.cpp", "xc_dispatch.cpp", "xc_serve.cpp", "xc_tiled.cpp"):
+    for name in ("xc_engine.cpp", "xc_dispatch.cpp", "xc_serve.cpp", "xc_tiled.cpp", "xc_hostio.cpp"):
         src = os.path.join(CSRC, name)
         out = os.path.join(OBJ, name.replace(".cpp", ".o"))
         hobjs.append(out)

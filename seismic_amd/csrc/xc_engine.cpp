@@ -19,6 +19,7 @@
 
 #include "../../include/seismic_xcorr.h"
 #include "xc_engine.h"
+#include "xc_hostio.h"
 #include "xc_kernels.h"
 #include "xc_lag.h"
 #include "xc_plan.h"
@@ -429,8 +430,125 @@ static int check_tie_overflow(sxc_plan* p) {
   return SXC_OK;
 }
 
+// Device state of one host-buffer job shape: the plan, the input and output
+// buffers and (OMP ABI) the histograms.  Kept between calls of the same shape
+// on the same device, so a caller that runs one shape repeatedly pays the
+// plan set-up and the allocations once.  Idle jobs are held up to a budget of
+// device memory (SEISMIC_XCORR_CACHE_MB, default 8192; 0 frees every job at
+// the end of its call, as the reference does); the least recently used are
+// freed first.  A job is used by one call at a time; concurrent calls of one
+// shape each get their own.
+struct HostJob {
+  sxc_plan_desc d{};
+  long long stride = 0;
+  bool reversed = false;
+  int n_classes = 0;
+  int device = -1;
+  size_t buf_bytes = 0;  // input, output and histogram buffers
+  size_t bytes = 0;      // with the plan's, as of the last release
+  sxc_plan* p = nullptr;
+  float *in = nullptr, *rev = nullptr, *vp = nullptr, *vn = nullptr;
+  int *lp = nullptr, *ln = nullptr, *hist = nullptr;
+};
+
+// Device bytes a plan holds (its large buffers; tables and counters aside).
+static size_t plan_device_bytes(const sxc_plan* p) {
+  const size_t rows = (size_t)p->d.n_events * p->d.n_channels;
+  size_t b = rows * (size_t)p->spec_stride * sizeof(float2) * ((p->spec ? 1 : 0) + (p->col_spec ? 1 : 0));
+  b += rows * sizeof(double) * ((p->norms ? 1 : 0) + (p->fnorms ? 1 : 0));
+  if (p->traces) b += rows * (size_t)p->P * sizeof(float);
+  b += (size_t)p->n_work * sizeof(int2) + (size_t)p->tie_cap * sizeof(int4) * (p->lag ? 3 : 1);
+  if (p->lag) b += rows * (size_t)p->ls * 2 * (p->lo ? 2 : 1) + rows * (size_t)p->d.trace_len * sizeof(float);
+  return b;
+}
+
+static void job_free(HostJob* j) {
+  if (!j) return;
+  void* bufs[] = {j->in, j->rev, j->vp, j->vn, j->lp, j->ln, j->hist};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  sxc_plan_destroy(j->p);
+  delete j;
+}
+
+static std::mutex g_jobs_mu;
+// never destroyed: device memory must not be freed by static destructors
+// after the HIP runtime has shut down at process exit
+static std::vector<HostJob*>* g_jobs = new std::vector<HostJob*>();  // idle, most recent last
+
+static size_t job_cache_budget() {
+  const char* e = getenv("SEISMIC_XCORR_CACHE_MB");
+  if (!e || !*e) return (size_t)8192 << 20;
+  const long long mb = atoll(e);
+  return mb > 0 ? (size_t)mb << 20 : 0;
+}
+
+static int job_acquire(const sxc_plan_desc& d, long long stride, bool reversed, int n_classes, HostJob** out) {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_jobs_mu);
+    for (size_t k = g_jobs->size(); k-- > 0;) {
+      HostJob* j = (*g_jobs)[k];
+      if (j->device == dev && j->stride == stride && j->reversed == reversed && j->n_classes == n_classes &&
+          memcmp(&j->d, &d, sizeof(d)) == 0) {
+        g_jobs->erase(g_jobs->begin() + (long)k);
+        *out = j;
+        return SXC_OK;
+      }
+    }
+  }
+  HostJob* j = new HostJob();
+  j->d = d;
+  j->stride = stride;
+  j->reversed = reversed;
+  j->n_classes = n_classes;
+  j->device = dev;
+  int rc = sxc_plan_create(&d, &j->p);
+  const long long rows = (long long)d.n_events * d.n_channels;
+  const size_t in = (size_t)(rows * stride), np = (size_t)d.n_events * (d.n_events + 1) / 2;
+  if (!rc) rc = dev_alloc(&j->in, in, "input traces");
+  if (!rc && reversed) rc = dev_alloc(&j->rev, in, "reversed traces");
+  if (!rc) rc = dev_alloc(&j->vp, np, "output values");
+  if (!rc) rc = dev_alloc(&j->vn, np, "output values");
+  if (!rc) rc = dev_alloc(&j->lp, np, "output lags");
+  if (!rc) rc = dev_alloc(&j->ln, np, "output lags");
+  if (!rc && n_classes > 0) rc = dev_alloc(&j->hist, 2 * (size_t)n_classes, "histograms");
+  if (rc) {
+    job_free(j);
+    return rc;
+  }
+  j->buf_bytes = (in * (reversed ? 2 : 1) + 4 * np + 2 * (size_t)n_classes) * 4;
+  *out = j;
+  return SXC_OK;
+}
+
+// Back to the cache (or freed); evicts the least recently used idle jobs
+// beyond the budget.
+static void job_release(HostJob* j) {
+  const size_t budget = job_cache_budget();
+  j->bytes = j->buf_bytes + plan_device_bytes(j->p);  // the plan may have grown (reversed-row spectra)
+  std::vector<HostJob*> drop;
+  {
+    std::lock_guard<std::mutex> lk(g_jobs_mu);
+    if (j->bytes <= budget) {
+      g_jobs->push_back(j);
+      j = nullptr;
+    }
+    size_t total = 0;
+    for (HostJob* x : *g_jobs) total += x->bytes;
+    while (total > budget && !g_jobs->empty()) {
+      total -= g_jobs->front()->bytes;
+      drop.push_back(g_jobs->front());
+      g_jobs->erase(g_jobs->begin());
+    }
+  }
+  if (j) drop.push_back(j);
+  for (HostJob* x : drop) job_free(x);
+}
+
 // Host-memory run: copy traces (and reversed rows) in, execute, check, copy
-// the packed outputs back, free.  The outputs are written even when near ties
+// the packed outputs back.  The outputs are written even when near ties
 // overflowed (the error is still returned).
 static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const float* reversed,
                          long long trace_stride, int in_len, float* vals_pos, int* lags_pos, float* vals_neg,
@@ -439,36 +557,21 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
     set_error("null argument");
     return SXC_ERR_INVALID;
   }
-  sxc_plan* p = nullptr;
-  int rc = sxc_plan_create(desc, &p);
+  int rc = validate_desc(*desc);
   if (rc) return rc;
+  HostJob* j = nullptr;
+  if ((rc = job_acquire(*desc, trace_stride, reversed != nullptr, 0, &j))) return rc;
   const long long rows = (long long)desc->n_events * desc->n_channels;
   const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
-  const size_t np = (size_t)p->n_pairs;
-  float *d_in = nullptr, *d_rev = nullptr, *d_vp = nullptr, *d_vn = nullptr;
-  int *d_lp = nullptr, *d_ln = nullptr;
-  auto cleanup = [&]() {
-    void* bufs[] = {d_in, d_rev, d_vp, d_vn, d_lp, d_ln};
-    for (void* b : bufs)
-      if (b) (void)hipFree(b);
-    sxc_plan_destroy(p);
-  };
-  if ((rc = dev_alloc(&d_in, (size_t)(rows * trace_stride), "input traces")) ||
-      (reversed && (rc = dev_alloc(&d_rev, (size_t)(rows * trace_stride), "reversed traces"))) ||
-      (rc = dev_alloc(&d_vp, np, "output values")) || (rc = dev_alloc(&d_vn, np, "output values")) ||
-      (rc = dev_alloc(&d_lp, np, "output lags")) || (rc = dev_alloc(&d_ln, np, "output lags"))) {
-    cleanup();
+  const size_t np = (size_t)j->p->n_pairs;
+  if ((rc = copy_to_device(j->in, traces, in_bytes, nullptr)) ||
+      (reversed && (rc = copy_to_device(j->rev, reversed, in_bytes, nullptr)))) {
+    job_release(j);
     return rc;
   }
-  hipError_t e = hipMemcpy(d_in, traces, in_bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess && reversed) e = hipMemcpy(d_rev, reversed, in_bytes, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    set_error("copying traces to the device failed: %s", hipGetErrorString(e));
-    cleanup();
-    return SXC_ERR_HIP;
-  }
-  rc = reversed ? sxc_plan_execute_reversed(p, d_in, d_rev, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr)
-                : sxc_plan_execute(p, d_in, trace_stride, in_len, d_vp, d_lp, d_vn, d_ln, nullptr);
+  rc = reversed ? sxc_plan_execute_reversed(j->p, j->in, j->rev, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln,
+                                            nullptr)
+                : sxc_plan_execute(j->p, j->in, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln, nullptr);
   if (rc == SXC_OK) {
     hipError_t es = hipDeviceSynchronize();
     if (es != hipSuccess) {
@@ -477,18 +580,19 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
     }
   }
   int tie_rc = SXC_OK;
-  if (rc == SXC_OK) tie_rc = check_tie_overflow(p);
+  if (rc == SXC_OK) tie_rc = check_tie_overflow(j->p);
   if (rc == SXC_OK) {
-    hipError_t e1 = hipMemcpy(vals_pos, d_vp, np * sizeof(float), hipMemcpyDeviceToHost);
-    hipError_t e2 = hipMemcpy(lags_pos, d_lp, np * sizeof(int), hipMemcpyDeviceToHost);
-    hipError_t e3 = hipMemcpy(vals_neg, d_vn, np * sizeof(float), hipMemcpyDeviceToHost);
-    hipError_t e4 = hipMemcpy(lags_neg, d_ln, np * sizeof(int), hipMemcpyDeviceToHost);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
-      set_error("copying results to the host failed");
-      rc = SXC_ERR_HIP;
-    }
+    int crc = copy_to_host(vals_pos, j->vp, np * sizeof(float), nullptr);
+    if (!crc) crc = copy_to_host(lags_pos, j->lp, np * sizeof(int), nullptr);
+    if (!crc) crc = copy_to_host(vals_neg, j->vn, np * sizeof(float), nullptr);
+    if (!crc) crc = copy_to_host(lags_neg, j->ln, np * sizeof(int), nullptr);
+    if (crc) rc = crc;
   }
-  cleanup();
+  if (rc == SXC_ERR_HIP) {
+    job_free(j);  // a failed device may have left the job's state unusable
+  } else {
+    job_release(j);
+  }
   return rc ? rc : tie_rc;
 }
 
@@ -878,64 +982,51 @@ void reference_call_omp(float* events, float* events_reversed, int n_events, int
     return;
   }
   d.threshold = -INFINITY;
-  sxc_plan* p = nullptr;
-  if (sxc_plan_create(&d, &p)) return;
-  float *d_in = nullptr, *d_rev = nullptr, *d_vp = nullptr, *d_vn = nullptr;
-  int *d_lp = nullptr, *d_ln = nullptr, *d_hist = nullptr;
-  auto cleanup = [&]() {
-    void* bufs[] = {d_in, d_rev, d_vp, d_vn, d_lp, d_ln, d_hist};
-    for (void* b : bufs)
-      if (b) (void)hipFree(b);
-    sxc_plan_destroy(p);
-  };
+  HostJob* j = nullptr;
+  if (job_acquire(d, padded, true, n_classes, &j)) return;
   const size_t n = (size_t)np;
-  if (dev_alloc(&d_in, (size_t)n_events * padded, "input traces") ||
-      dev_alloc(&d_rev, (size_t)n_events * padded, "reversed traces") || dev_alloc(&d_vp, n, "output values") ||
-      dev_alloc(&d_vn, n, "output values") || dev_alloc(&d_lp, n, "output lags") ||
-      dev_alloc(&d_ln, n, "output lags") || dev_alloc(&d_hist, 2 * (size_t)n_classes, "histograms")) {
-    cleanup();
-    return;
-  }
   std::vector<int> h(2 * (size_t)n_classes);
-  bool ok = hipMemcpy(d_in, events, (size_t)n_events * padded * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
-            hipMemcpy(d_rev, events_reversed, (size_t)n_events * padded * sizeof(float), hipMemcpyHostToDevice) ==
-                hipSuccess &&
-            hipMemset(d_hist, 0, 2 * (size_t)n_classes * sizeof(int)) == hipSuccess;
+  const size_t in_bytes = (size_t)n_events * padded * sizeof(float);
+  bool ok = copy_to_device(j->in, events, in_bytes, nullptr) == SXC_OK &&
+            copy_to_device(j->rev, events_reversed, in_bytes, nullptr) == SXC_OK &&
+            hipMemset(j->hist, 0, 2 * (size_t)n_classes * sizeof(int)) == hipSuccess;
   if (ok)
-    ok = sxc_plan_execute_reversed(p, d_in, d_rev, padded, padded, d_vp, d_lp, d_vn, d_ln, nullptr) == SXC_OK;
+    ok = sxc_plan_execute_reversed(j->p, j->in, j->rev, padded, padded, j->vp, j->lp, j->vn, j->ln, nullptr) ==
+         SXC_OK;
   int tie_rc = SXC_OK;
   if (ok) {
     HistArgs ha{};
-    ha.vpos = d_vp;
-    ha.lpos = d_lp;
-    ha.vneg = d_vn;
-    ha.lneg = d_ln;
+    ha.vpos = j->vp;
+    ha.lpos = j->lp;
+    ha.vneg = j->vn;
+    ha.lneg = j->ln;
     ha.n = np;
     ha.n_classes = n_classes;
     ha.delta = (float)(1.0 / n_classes);
     ha.threshold = threshold;
-    ha.hist_pos = d_hist;
-    ha.hist_neg = d_hist + n_classes;
+    ha.hist_pos = j->hist;
+    ha.hist_neg = j->hist + n_classes;
     ok = launch_hist_threshold(ha, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     if (!ok) set_error("histogram pass failed");
-    if (ok) tie_rc = check_tie_overflow(p);
+    if (ok) tie_rc = check_tie_overflow(j->p);
   }
   if (ok)
-    ok = hipMemcpy(vp, d_vp, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(lp, d_lp, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(vn, d_vn, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(ln, d_ln, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(h.data(), d_hist, h.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    ok = copy_to_host(vp, j->vp, n * sizeof(float), nullptr) == SXC_OK &&
+         copy_to_host(lp, j->lp, n * sizeof(int), nullptr) == SXC_OK &&
+         copy_to_host(vn, j->vn, n * sizeof(float), nullptr) == SXC_OK &&
+         copy_to_host(ln, j->ln, n * sizeof(int), nullptr) == SXC_OK &&
+         hipMemcpy(h.data(), j->hist, h.size() * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
   if (ok) {
     for (int k = 0; k < n_classes; ++k) {
       max_hist[k] += h[k];
       min_hist[k] += h[n_classes + k];
     }
-  } else if (!last_error()[0]) {
-    set_error("device execution or copy failed");
+    job_release(j);
+  } else {
+    if (!last_error()[0]) set_error("device execution or copy failed");
+    job_free(j);
   }
   (void)tie_rc;  // an overflow has set the error string; the results are delivered
-  cleanup();
 }
 
 int warm_up() {

@@ -426,3 +426,34 @@ def test_dropin_concurrent_host_threads():
     for a, b in zip(alone, got):
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
+
+
+def test_dropin_kept_device_state_is_reset(monkeypatch):
+    """Host-buffer calls keep the device state of their shape between calls
+    (plan, input and output buffers).  A reused job must give exactly what a
+    fresh one gives: the same shape called on other data, on other data with
+    a threshold, and interleaved with another shape, against the same calls
+    with the cache off (SEISMIC_XCORR_CACHE_MB=0: every job freed at the end
+    of its call).  Below-threshold entries and histograms (OMP ABI) must not
+    carry over from the previous call either."""
+    from seismic_amd import omp_lib
+    evs = [synth.make_events(60, 1, 1500, seed=300 + t)[0][:, 0] for t in range(3)]
+    other = synth.make_events(33, 1, 700, seed=310)[0][:, 0]
+
+    def run_all():
+        out = []
+        for k, e in enumerate(evs):
+            out.append(_run_dropin(e, 300, 0.3 if k == 1 else 0.0))
+            out.append(_run_dropin(other, 200, 0.0))
+            out.append(omp_lib.compute_correlation(e, 300, 1, 1, 0.2, 16))
+        return out
+
+    print("case drop-in kept device state", flush=True)
+    monkeypatch.setenv("SEISMIC_XCORR_CACHE_MB", "0")
+    fresh = run_all()
+    monkeypatch.delenv("SEISMIC_XCORR_CACHE_MB")
+    kept = run_all()
+    kept2 = run_all()
+    for a, b, c in zip(fresh, kept, kept2):
+        for u, v, w in zip(a, b, c):
+            assert np.array_equal(u, v) and np.array_equal(u, w)
