@@ -106,27 +106,34 @@ __global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
 // L samples in FP64, sums the energy of the first L samples in FP64, and
 // writes Xhat[k] = rfft(x)[k] * scale for k = 0..M, scale = 1/sqrt(E*C*P)
 // (0 for a zero-energy row).
+// A workgroup of fwd_waves<R>() waves stages the twiddle tables once and
+// walks rows (blockIdx.x + g gridDim.x) W + wave; each wave keeps one buffer
+// of max(kWaveLds, M) entries for its transposes and, after its FFT, for the
+// natural-order Z (in-order LDS accesses within a wave let the two share it).
 template <int R>
-__global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
+constexpr int fwd_wave_buf() {
+  return 64 * R > kWaveLds ? 64 * R : kWaveLds;
+}
+template <int R>
+constexpr int fwd_waves() {
+  constexpr int w = (160 * 1024 - TwLayout<R>::kFft * 8) / (fwd_wave_buf<R>() * 8);
+  return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
+template <int R>
+__device__ __forceinline__ void forward_row(const ForwardArgs& a, int row, const float2* twl, float2* wl,
+                                            float2* zl, int lane) {
   constexpr int M = 64 * R;
   constexpr int P = 2 * M;
-  extern __shared__ float2 smem[];
-  float2* twl = smem;                            // twiddle tables
-  float2* wl = smem + TwLayout<R>::kFft;         // wave FFT transposes
-  float2* zl = wl + kWaveLds;                    // natural-order Z, M entries
-  const int lane = threadIdx.x;
-  const int row = blockIdx.x;
-  if (row >= a.n_rows) return;
-  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kFft, lane, 64);
-  wave_lds_fence();
-
+  wave_lds_fence();  // the previous row's reads of this wave's buffer come first
   const float* x = a.in + (long long)row * a.in_stride;
+  int in_len = a.in_len, L = a.L;
   if (a.row_len) {  // uneven lengths: this row's event sets both counts
     const int n = a.row_len[row / a.n_channels];
-    a.in_len = n < (int)a.in_stride ? n : (int)a.in_stride;
-    a.L = a.in_len;
+    in_len = n < (int)a.in_stride ? n : (int)a.in_stride;
+    L = in_len;
   }
-  if (a.in_len < 1) {  // empty row: zero spectrum
+  if (in_len < 1) {  // empty row: zero spectrum
     for (int k = lane; k <= M; k += 64) a.spec[(long long)row * a.spec_stride + k] = make_float2(0.f, 0.f);
     if (lane == 0 && !a.reversed) a.norms[row] = 0.0;
     return;
@@ -139,11 +146,11 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
     E = nr * nr;
   } else {
     double s1 = 0.0;
-    for (int n = lane; n < a.L; n += 64) s1 += (double)(n < a.in_len ? x[n] : 0.f);
-    mean = a.demean ? wave_sum_f64(s1) / (double)a.L : 0.0;
+    for (int n = lane; n < L; n += 64) s1 += (double)(n < in_len ? x[n] : 0.f);
+    mean = a.demean ? wave_sum_f64(s1) / (double)L : 0.0;
     double s2 = 0.0;
-    for (int n = lane; n < a.L; n += 64) {
-      const float v = n < a.in_len ? x[n] : 0.f;
+    for (int n = lane; n < L; n += 64) {
+      const float v = n < in_len ? x[n] : 0.f;
       const float u = a.demean ? (float)((double)v - mean) : v;
       s2 += (double)u * (double)u;
     }
@@ -151,21 +158,21 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
   }
   // d[m] = x[2(t+64m)] + i x[2(t+64m)+1]; clamped loads + selects, no branches
   cf d[R];
-  const int last = a.in_len > 0 ? a.in_len - 1 : 0;
+  const int last = in_len > 0 ? in_len - 1 : 0;
 #pragma unroll
   for (int m = 0; m < R; ++m) {
     const int n = 2 * (lane + 64 * m);
     const float e0 = x[min(n, last)], e1 = x[min(n + 1, last)];
-    d[m].x = n < a.in_len ? e0 : 0.f;
-    d[m].y = n + 1 < a.in_len ? e1 : 0.f;
+    d[m].x = n < in_len ? e0 : 0.f;
+    d[m].y = n + 1 < in_len ? e1 : 0.f;
   }
   if (a.demean) {
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int n = 2 * (lane + 64 * m);
       const float u0 = (float)((double)d[m].x - mean), u1 = (float)((double)d[m].y - mean);
-      d[m].x = n < a.L ? u0 : d[m].x;
-      d[m].y = n + 1 < a.L ? u1 : d[m].y;
+      d[m].x = n < L ? u0 : d[m].x;
+      d[m].y = n + 1 < L ? u1 : d[m].y;
     }
   }
   if (a.out_traces) {
@@ -206,6 +213,25 @@ __global__ __launch_bounds__(64) void k_forward(ForwardArgs a) {
     }
     if (k == 0 || k == M) X.y = 0.f;
     out[k] = make_float2(X.x * scale, X.y * scale);
+  }
+}
+
+
+template <int R>
+__global__ __launch_bounds__(64 * fwd_waves<R>()) void k_forward(ForwardArgs a) {
+  constexpr int M = 64 * R;
+  constexpr int P = 2 * M;
+  constexpr int W = fwd_waves<R>();
+  extern __shared__ float2 smem[];
+  float2* twl = smem;  // twiddle tables
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  float2* wl = smem + TwLayout<R>::kFft + wv * fwd_wave_buf<R>();  // this wave's transposes, then Z
+  float2* zl = wl;
+  stage_twiddles<R>(twl, a.tw, TwLayout<R>::kFft, threadIdx.x, 64 * W);
+  __syncthreads();
+  for (int row = blockIdx.x * W + wv; row < a.n_rows; row += gridDim.x * W) {
+    forward_row<R>(a, row, twl, wl, zl, lane);
   }
 }
 
@@ -664,11 +690,13 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
 // ----------------------------------------------------------------------------
 template <int R>
 hipError_t launch_forward_r(const ForwardArgs& a, hipStream_t st) {
-  constexpr int M = 64 * R;
-  size_t lds = (size_t)(TwLayout<R>::kFft + kWaveLds + M) * sizeof(float2);
+  if (a.n_rows <= 0) return hipSuccess;
+  constexpr int W = fwd_waves<R>();
+  const size_t lds = (size_t)(TwLayout<R>::kFft + W * fwd_wave_buf<R>()) * sizeof(float2);
   hipError_t e = hipFuncSetAttribute((const void*)k_forward<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_forward<R>, dim3(a.n_rows), dim3(64), lds, st, a);
+  const int groups = (a.n_rows + W - 1) / W;
+  hipLaunchKernelGGL(k_forward<R>, dim3(groups < 1024 ? groups : 1024), dim3(64 * W), lds, st, a);
   return hipGetLastError();
 }
 
