@@ -547,16 +547,28 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
 // value is within tol of the winner, evaluates those lags as FP64 direct
 // sums over the time-domain traces and keeps the largest (smallest) with
 // ties to the smaller lag; starting state (0, lag 0) as in the reference scan.
+// (four independent partial sums per lane, so the loads and FP64 FMAs of
+// successive samples overlap instead of forming one dependent chain)
 __device__ double corr_f64(const float* __restrict__ ti, const float* __restrict__ tj, int len,
                            int P, int lag, int lane) {
-  double s = 0.0;
-  for (int n = lane; n < len; n += 64) {
+  auto term = [&](int n) -> double {
     int u = n - lag;  // |lag| <= P/2 and n < P, so one wrap suffices
     u += u < 0 ? P : 0;
     u -= u >= P ? P : 0;
-    if (u < len) s += (double)ti[n] * (double)tj[u];
+    const bool ok = u < len;
+    const float b = tj[ok ? u : 0];  // always in bounds
+    return ok ? (double)ti[n] * (double)b : 0.0;
+  };
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int n = lane;
+  for (; n + 192 < len; n += 256) {
+    s0 += term(n);
+    s1 += term(n + 64);
+    s2 += term(n + 128);
+    s3 += term(n + 192);
   }
-  return wave_sum_f64(s);
+  for (; n < len; n += 64) s0 += term(n);
+  return wave_sum_f64((s0 + s1) + (s2 + s3));
 }
 
 // The reference's own sum for reversed rows (Correlation_in_OMP/
@@ -566,13 +578,23 @@ __device__ double conv_f64(const float* __restrict__ ti, int len_i, const float*
                            int P, int q, int lane) {
   q %= P;
   q += q < 0 ? P : 0;
-  double s = 0.0;
-  for (int m = lane; m < len_i; m += 64) {
+  auto term = [&](int m) -> double {
     int u = q - m;  // q, m in [0, P): one wrap suffices
     u += u < 0 ? P : 0;
-    if (u < len_j) s += (double)ti[m] * (double)rj[u];
+    const bool ok = u < len_j;
+    const float b = rj[ok ? u : 0];  // always in bounds
+    return ok ? (double)ti[m] * (double)b : 0.0;
+  };
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int m = lane;
+  for (; m + 192 < len_i; m += 256) {
+    s0 += term(m);
+    s1 += term(m + 64);
+    s2 += term(m + 128);
+    s3 += term(m + 192);
   }
-  return wave_sum_f64(s);
+  for (; m < len_i; m += 64) s0 += term(m);
+  return wave_sum_f64((s0 + s1) + (s2 + s3));
 }
 
 template <int R>
