@@ -349,27 +349,40 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
 // ----------------------------------------------------------------------------
 // FP64 resolver for the lag-window form: one wave per recorded pair
 // ----------------------------------------------------------------------------
-__device__ __forceinline__ double lin_corr_f64(const float* __restrict__ ti, const float* __restrict__ tj,
-                                               int L, int lag, int lane) {
-  // four samples per lane per trip, loads issued together, four accumulators
-  const int n0 = lag > 0 ? lag : 0;
-  const int n1 = lag < 0 ? L + lag : L;
-  auto term = [&](int n) -> double {
-    const bool ok = n < n1;
-    const float a = ti[ok ? n : n0], b = tj[ok ? n - lag : n0 - lag];
-    return ok ? (double)a * (double)b : 0.0;
-  };
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int n = n0 + lane; n < n1; n += 256) {
-    s0 += term(n);
-    s1 += term(n + 64);
-    s2 += term(n + 128);
-    s3 += term(n + 192);
-  }
-  double s = (s0 + s1) + (s2 + s3);
+
+// FP64 sums sum_n ti[n] tj[n - lag_q] over the overlap of the two traces in
+// [0, L), for four lags at once (callers repeat a lag to fill unused slots):
+// each ti sample is read once for all of them, the loads of all four are
+// issued together (no per-lag branches), and two sets of partial sums per
+// lane keep the FP64 FMA chains short.  The results are wave-uniform.
+__device__ __forceinline__ void lin_corr4_f64(const float* __restrict__ ti, const float* __restrict__ tj, int L,
+                                              const int (&lag)[4], int lane, double (&out)[4]) {
+  double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
+  auto step = [&](int n, double (&acc)[4]) {
+    const bool in = n < L;
+    const double a = (double)ti[in ? n : 0];
+    float b[4];
+    bool ok[4];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  return s;
+    for (int q = 0; q < 4; ++q) {
+      const int u = n - lag[q];
+      ok[q] = in && u >= 0 && u < L;
+      b[q] = tj[ok[q] ? u : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += ok[q] ? a * (double)b[q] : 0.0;
+  };
+  for (int n = lane; n < L; n += 128) {
+    step(n, sa);
+    step(n + 64, sb);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double v = sa[q] + sb[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    out[q] = v;
+  }
 }
 
 // Every lag of [lag_lo, lag_hi] at once, one lag per lane: FP64 sum of
@@ -446,33 +459,56 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
       const unsigned slot = atomicAdd(a.wide_count, 1u);
       if (slot < (unsigned)a.wide_cap) a.wide[slot] = make_int4(i, j, wide_sides, 0);
     }
+    // candidate lags of the sides resolved here (the two recorded ones per
+    // side, in the window, without repeats), evaluated in one pass over the
+    // traces per channel
+    int lag[4] = {0, 0, 0, 0}, q_side[4] = {0, 0, 0, 0};
+    int nq = 0;
+    bool sweep[2] = {false, false};
     for (int sd = 0; sd < 2; ++sd) {
       if (!(sides & (1 << sd))) continue;
+      if (sides & (4 << sd)) {
+        sweep[sd] = !a.wide;  // else handed to the FFT form's resolver
+        continue;
+      }
+      const int c0 = __builtin_amdgcn_readfirstlane(sd == 0 ? t1.x : t1.z);
+      const int c1 = __builtin_amdgcn_readfirstlane(sd == 0 ? t1.y : t1.w);
+      if (c0 >= a.lag_lo && c0 <= a.lag_hi) { lag[nq] = c0; q_side[nq] = sd; ++nq; }
+      if (c1 != c0 && c1 >= a.lag_lo && c1 <= a.lag_hi) { lag[nq] = c1; q_side[nq] = sd; ++nq; }
+    }
+    for (int q = nq; q < 4; ++q) lag[q] = lag[0];  // unused slots repeat a lag
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (nq > 0) {
+      for (int c = 0; c < a.n_channels; ++c) {
+        const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
+        if (!(ni > 0.0 && nj > 0.0)) continue;
+        const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+        const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+        double sum[4];
+        lin_corr4_f64(ti, tj, a.L, lag, lane, sum);
+        const double w = ni * nj * (double)a.n_channels;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] += sum[q] / w;
+      }
+    }
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!(sides & (1 << sd))) continue;
+      if ((sides & (4 << sd)) && !sweep[sd]) continue;
       const double sg = sd == 0 ? 1.0 : -1.0;
       double best = 0.0;
       int best_lag = 0;
       bool have = false;
-      if (sides & (4 << sd)) {
-        if (a.wide) continue;  // handed to the FFT form's resolver
+      if (sweep[sd]) {
         sweep_window_f64(a, i, j, sg, lane, best, best_lag, have);
       } else {
         // only the two recorded lags can hold the FP64 winner
-        const int c0 = sd == 0 ? t1.x : t1.z, c1 = sd == 0 ? t1.y : t1.w;
-        for (int q = 0; q < 2; ++q) {
-          const int lag = q == 0 ? c0 : c1;
-          if (lag < a.lag_lo || lag > a.lag_hi || (q == 1 && c1 == c0)) continue;
-          double acc = 0.0;
-          for (int c = 0; c < a.n_channels; ++c) {
-            const double ni = a.norms[(long long)i * a.n_channels + c], nj = a.norms[(long long)j * a.n_channels + c];
-            if (!(ni > 0.0 && nj > 0.0)) continue;
-            const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
-            const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
-            acc += lin_corr_f64(ti, tj, a.L, lag, lane) / (ni * nj * (double)a.n_channels);
-          }
-          acc *= sg;
-          if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
-            best = acc;
-            best_lag = lag;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (q >= nq || q_side[q] != sd) continue;
+          const double v = sg * acc[q];
+          if (v > 0.0 && (!have || v > best || (v == best && lag[q] < best_lag))) {
+            best = v;
+            best_lag = lag[q];
             have = true;
           }
         }
