@@ -126,7 +126,9 @@ struct Stager {
 };
 
 std::mutex g_mu;
-std::vector<std::unique_ptr<Stager>> g_idle;  // stagers not in use
+// stagers not in use; never destroyed, so no pinned memory is freed by a
+// static destructor after the HIP runtime has shut down at process exit
+std::vector<std::unique_ptr<Stager>>* g_idle = new std::vector<std::unique_ptr<Stager>>();
 
 // An idle stager of the current device, or a new one; nullptr on failure
 // (error string set).
@@ -138,10 +140,10 @@ std::unique_ptr<Stager> acquire() {
   }
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    for (size_t k = 0; k < g_idle.size(); ++k)
-      if (g_idle[k]->device == dev) {
-        std::unique_ptr<Stager> s = std::move(g_idle[k]);
-        g_idle.erase(g_idle.begin() + (long)k);
+    for (size_t k = 0; k < g_idle->size(); ++k)
+      if ((*g_idle)[k]->device == dev) {
+        std::unique_ptr<Stager> s = std::move((*g_idle)[k]);
+        g_idle->erase(g_idle->begin() + (long)k);
         return s;
       }
   }
@@ -159,7 +161,7 @@ std::unique_ptr<Stager> acquire() {
 
 void release(std::unique_ptr<Stager> s) {
   std::lock_guard<std::mutex> lk(g_mu);
-  if (g_idle.size() < 4) g_idle.push_back(std::move(s));
+  if (g_idle->size() < 4) g_idle->push_back(std::move(s));
 }
 
 }  // namespace
