@@ -200,7 +200,17 @@ __device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsign
   for (int st = 0; st < total; st += kDepth) {
 #pragma unroll
     for (int q = 0; q < kDepth; ++q) {
-      if (st + q < total) mfma_step<BF16X3>(acc, buf[q]);
+      if (st + q < total) {
+        mfma_step<BF16X3>(acc, buf[q]);
+        // scheduling: the operand assembly (v_mov, v_alignbyte) spread
+        // evenly between the MFMAs, so the single wave keeps issuing while
+        // its MFMAs run (fp16 4.6% faster, bf16x3 0.3%; same instructions)
+#pragma unroll
+        for (int g = 0; g < (BF16X3 ? 48 : 16); ++g) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+          __builtin_amdgcn_sched_group_barrier(0x002, BF16X3 ? 3 : 5, 0);  // then up to this many VALU
+        }
+      }
       // refill (clamped at the end: a repeated in-bounds load is harmless)
       load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ls, spc, min(st + q + kDepth, total - 1), p0, lb, h);
     }
