@@ -17,7 +17,9 @@
 //           is hi*hi + hi*lo + lo*hi with FP32 accumulation (3 MFMAs);
 //   fp16:   one FP16 product, FP32 accumulation.
 // FP64 is used for the per-trace energy and scale, and for the near-tie
-// re-evaluation (k_lag_resolve).  Scan rule as in the reference
+// re-evaluation (k_lag_resolve at the two recorded lags; records with a
+// third lag inside the band go to the FFT form's resolver, which finds every
+// candidate from the pair's FP32 transform).  Scan rule as in the reference
 // (Correlation_in_OMP/correlation_c.cpp:102-136): pos = largest value above
 // 0 at its first lag, neg = smallest below 0, threshold to (0, 0).
 #include <hip/hip_runtime.h>
