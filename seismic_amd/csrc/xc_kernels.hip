@@ -251,6 +251,23 @@ constexpr int load_group(int R, int cap) {
   return g;
 }
 
+// Spectrum rows read from global memory go through a buffer resource: the
+// row base is a scalar, one per-lane offset register (8 lane) serves a whole
+// 4 KiB stretch through the instruction's immediate offset, and a channel's
+// start is the scalar offset, so the loads need no per-pair vector address
+// arithmetic.  `base` must be wave-uniform; a row of nch channels spans
+// ((nch - 1) stride + M + 1) float2, at most 8 channels of 4097 bins (the
+// plans' limits), far below the 2^31 bytes a resource can describe.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t spec_rsrc(const float2* base, long long stride, int nch, int M) {
+  const long long bytes = 8 * ((long long)(nch - 1) * stride + M + 1);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 spec_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
 template <int R, bool XI_GLOBAL = false, bool CT = false, int kLoadGroup = 32>
 __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restrict__ xi,
                                              int xi_stride, const float2* __restrict__ xj,
@@ -258,11 +275,19 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
                                              float2* wl, int lane) {
   constexpr int M = 64 * R;
   float yN = 0.f;
+  const __amdgpu_buffer_rsrc_t rj = spec_rsrc(xj, xj_stride, nch, M);
+  const __amdgpu_buffer_rsrc_t ri = spec_rsrc(xi, xi_stride, XI_GLOBAL ? nch : 1, M);
   // channel 0 initialises the accumulators, channels 1.. add to them
   auto channel = [&](int c, auto first_c) {
     constexpr bool kFirst = decltype(first_c)::value;
     const float2* ai = xi + c * xi_stride;
-    const float2* bj = xj + c * xj_stride;
+    const int soff_j = (int)(8 * c * xj_stride), soff_i = 8 * c * xi_stride;
+    // per-lane byte offset, redefined per channel (in the loads' block, with
+    // a known range) so the bin offsets stay instruction immediates instead
+    // of one hoisted offset register per load
+    int lv = lane;
+    asm volatile("" : "+v"(lv));
+    const int vb = 8 * (lv & 63);
     // All of a group's global loads are issued before the compiler fence so
     // they are in flight together (one L2 round trip per group, not per row).
     constexpr int kGroup = load_group(R, kLoadGroup);
@@ -272,10 +297,10 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
       float2 Bv[kGroup];
       float2 Av[XI_GLOBAL ? kGroup : 1];
 #pragma unroll
-      for (int u = 0; u < kGroup; ++u) Bv[u] = bj[lane + 64 * (m0 + u)];
+      for (int u = 0; u < kGroup; ++u) Bv[u] = spec_load(rj, vb + 512 * (m0 + u), soff_j);
       if constexpr (XI_GLOBAL) {
 #pragma unroll
-        for (int u = 0; u < kGroup; ++u) Av[u] = ai[lane + 64 * (m0 + u)];
+        for (int u = 0; u < kGroup; ++u) Av[u] = spec_load(ri, vb + 512 * (m0 + u), soff_i);
       }
       asm volatile("" ::: "memory");
 #pragma unroll
@@ -292,7 +317,7 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
         }
       }
     }
-    float2 aN = ai[M], bN = bj[M];
+    const float2 aN = ai[M], bN = xj[c * xj_stride + M];
     yN = kFirst ? __builtin_fmaf(aN.x, bN.x, aN.y * bN.y) : __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
   };
   channel(0, std::true_type{});
@@ -317,15 +342,29 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     return cmk(__builtin_fmaf(-t.x, dy, __builtin_fmaf(-t.y, dx, ex)),
                __builtin_fmaf(t.x, dx, __builtin_fmaf(-t.y, dy, ey)));
   };
+  // The mirror values travel by ds_bpermute (lane t <- lane 64 - t); those of
+  // iteration m + kAhead are requested before iteration m is computed, so the
+  // LDS round trips overlap the arithmetic instead of ending each iteration.
+  constexpr int kAhead = 2;
+  cf sh_b[kAhead], sh_a[kAhead];  // shuffled y[R - 1 - m], y[m] of iterations m .. m + kAhead - 1
+  auto request = [&](auto mc, cf& rb, cf& ra) {
+    constexpr int m = decltype(mc)::value;
+    rb.x = __shfl(y[R - 1 - m].x, src, 64);
+    rb.y = __shfl(y[R - 1 - m].y, src, 64);
+    ra.x = __shfl(y[m].x, src, 64);
+    ra.y = __shfl(y[m].y, src, 64);
+  };
+  sfor<0, (kAhead < R / 2 ? kAhead : R / 2)>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    request(qc, sh_b[q], sh_a[q]);
+  });
   sfor<0, R / 2>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     constexpr int mh = R - 1 - m;
+    constexpr int slot = m % kAhead;
     cf a = y[m], b = y[mh];
-    cf am, bm;
-    am.x = __shfl(b.x, src, 64);
-    am.y = __shfl(b.y, src, 64);
-    bm.x = __shfl(a.x, src, 64);
-    bm.y = __shfl(a.y, src, 64);
+    cf am = sh_b[slot], bm = sh_a[slot];
+    if constexpr (m + kAhead < R / 2) request(std::integral_constant<int, m + kAhead>{}, sh_b[slot], sh_a[slot]);
     {
       const cf nb = (m + 1 == mh) ? b : y[m + 1];
       const bool l0 = lane == 0;
@@ -403,25 +442,26 @@ __device__ __forceinline__ void wave_maxmin(float& mx, float& mn) {
 
 // Reference scan rule over the wave's values: strict > (<) starting from
 // (0, lag 0), first occurrence.  Pass 1 keeps the max (min) of each group of
-// two registers (4 values) and reduces them to the exact extrema (DPP wave
+// four registers (8 values) and reduces them to the exact extrema (DPP wave
 // reduction); pass 2 ballots each group maximum (minimum) against extremum
 // -/+ tol, and only for the rare groups where that ballot is non-zero
-// ballots the group's four values separately: the ballot counts are summed on
-// the scalar unit, and the last slot whose ballot is non-zero (with its lane
-// mask) locates the extremum.  More than one value within tol flags a near
-// tie, which the FP64 resolver settles.
+// ballots the group's eight values separately: the ballot counts are summed
+// on the scalar unit, and the last slot whose ballot is non-zero (with its
+// lane mask) locates the extremum.  More than one value within tol flags a
+// near tie, which the FP64 resolver settles.
 template <int R>
 __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol, float thr,
                                           Best& pos, Best& neg, bool& tie_p, bool& tie_n) {
-  static_assert(R % 2 == 0, "groups of two registers");
-  constexpr int G = R / 2;
+  static_assert(R % 4 == 0, "groups of four registers");
+  constexpr int G = R / 4;
   float gp[G], gn[G];
   float mx = 0.f, mn = 0.f;
   sfor<0, G>([&](auto gc) {
     constexpr int g = decltype(gc)::value;
-    const cf a = y[2 * g], b = y[2 * g + 1];
-    gp[g] = fmaxf(fmaxf(a.x, a.y), fmaxf(b.x, b.y));
-    gn[g] = fminf(fminf(a.x, a.y), fminf(b.x, b.y));
+    const cf a = y[4 * g], b = y[4 * g + 1], c = y[4 * g + 2], d = y[4 * g + 3];
+    // chained so that each three-operand max (min) takes two new values
+    gp[g] = fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(fmaxf(a.x, a.y), b.x), b.y), c.x), c.y), d.x), d.y);
+    gn[g] = fminf(fminf(fminf(fminf(fminf(fminf(fminf(a.x, a.y), b.x), b.y), c.x), c.y), d.x), d.y);
     mx = fmaxf(mx, gp[g]);
     mn = fminf(mn, gn[g]);
   });
@@ -435,21 +475,21 @@ __device__ __forceinline__ void scan_wave(const cf (&y)[R], int lane, float tol,
     sfor<0, G>([&](auto gc) {
       constexpr int g = decltype(gc)::value;
       if (__ballot(gp[g] >= cp)) {
-        sfor<0, 4>([&](auto qc) {
+        sfor<0, 8>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
-          const float v = (q & 1) ? y[2 * g + (q >> 1)].y : y[2 * g + (q >> 1)].x;
+          const float v = (q & 1) ? y[4 * g + (q >> 1)].y : y[4 * g + (q >> 1)].x;
           const unsigned long long b = __ballot(v >= cp);
           cnt_p += __popcll(b);
-          if (b) { slot_p = 4 * g + q; msk_p = b; }
+          if (b) { slot_p = 8 * g + q; msk_p = b; }
         });
       }
       if (__ballot(gn[g] <= cn)) {
-        sfor<0, 4>([&](auto qc) {
+        sfor<0, 8>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
-          const float v = (q & 1) ? y[2 * g + (q >> 1)].y : y[2 * g + (q >> 1)].x;
+          const float v = (q & 1) ? y[4 * g + (q >> 1)].y : y[4 * g + (q >> 1)].x;
           const unsigned long long b = __ballot(v <= cn);
           cnt_n += __popcll(b);
-          if (b) { slot_n = 4 * g + q; msk_n = b; }
+          if (b) { slot_n = 8 * g + q; msk_n = b; }
         });
       }
     });
@@ -480,7 +520,9 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
   constexpr int kTab = CT ? TwLayoutCT<R>::kInv : TwLayout<R>::kInv;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  // wave index as a scalar: column indices and spectrum addresses then stay
+  // in scalar registers (loads use a scalar base and a per-lane offset)
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int xs = XI_LDS ? M + 1 : (int)a.spec_stride;  // row stride of the row spectra
   float2* twl = smem;
   float2* xi_lds = smem + kTab;
@@ -547,54 +589,44 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
 // value is within tol of the winner, evaluates those lags as FP64 direct
 // sums over the time-domain traces and keeps the largest (smallest) with
 // ties to the smaller lag; starting state (0, lag 0) as in the reference scan.
-// (four independent partial sums per lane, so the loads and FP64 FMAs of
-// successive samples overlap instead of forming one dependent chain)
-__device__ double corr_f64(const float* __restrict__ ti, const float* __restrict__ tj, int len,
-                           int P, int lag, int lane) {
-  auto term = [&](int n) -> double {
-    int u = n - lag;  // |lag| <= P/2 and n < P, so one wrap suffices
-    u += u < 0 ? P : 0;
-    u -= u >= P ? P : 0;
-    const bool ok = u < len;
-    const float b = tj[ok ? u : 0];  // always in bounds
-    return ok ? (double)ti[n] * (double)b : 0.0;
+//
+// FP64 sums for four lags at once (callers repeat one to fill unused slots),
+// as shifts in [0, P):
+//   conv = false: sum_n ti[n] tj[(n - shift_q) mod P], the circular
+//                 correlation at lag shift_q;
+//   conv = true:  sum_m ti[m] tj[(shift_q - m) mod P], the circular
+//                 convolution with a reversed row, which is the reference's
+//                 own sum for reversed rows (Correlation_in_OMP/
+//                 correlation_c.cpp:270, :63, :102-103): its lag-l value is
+//                 corr_t[L - 1 + l], this sum at shift_q = L - 1 + l.
+// Each ti sample is read once for all four, their loads are issued together,
+// and two sets of partial sums per lane keep the FP64 FMA chains short.  The
+// results are wave-uniform.
+__device__ __forceinline__ void corr4_f64(const float* __restrict__ ti, int len_i, const float* __restrict__ tj,
+                                          int len_j, int P, const int (&shift)[4], bool conv, int lane,
+                                          double (&out)[4]) {
+  double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
+  auto step = [&](int n, double (&acc)[4]) {
+    const bool in = n < len_i;
+    const double x = (double)ti[in ? n : 0];
+    float b[4];
+    bool ok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int u = conv ? shift[q] - n : n - shift[q];  // n, shift in [0, P): one wrap suffices
+      u += u < 0 ? P : 0;
+      ok[q] = in && u < len_j;
+      b[q] = tj[ok[q] ? u : 0];  // always in bounds
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] += ok[q] ? x * (double)b[q] : 0.0;
   };
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int n = lane;
-  for (; n + 192 < len; n += 256) {
-    s0 += term(n);
-    s1 += term(n + 64);
-    s2 += term(n + 128);
-    s3 += term(n + 192);
+  for (int n = lane; n < len_i; n += 128) {
+    step(n, sa);
+    step(n + 64, sb);
   }
-  for (; n < len; n += 64) s0 += term(n);
-  return wave_sum_f64((s0 + s1) + (s2 + s3));
-}
-
-// The reference's own sum for reversed rows (Correlation_in_OMP/
-// correlation_c.cpp:270, :63, :102-103): its lag-l value is corr_t[L - 1 + l],
-// the circular convolution sum_m x_i[m] r_j[(q - m) mod P] at q = L - 1 + l.
-__device__ double conv_f64(const float* __restrict__ ti, int len_i, const float* __restrict__ rj, int len_j,
-                           int P, int q, int lane) {
-  q %= P;
-  q += q < 0 ? P : 0;
-  auto term = [&](int m) -> double {
-    int u = q - m;  // q, m in [0, P): one wrap suffices
-    u += u < 0 ? P : 0;
-    const bool ok = u < len_j;
-    const float b = rj[ok ? u : 0];  // always in bounds
-    return ok ? (double)ti[m] * (double)b : 0.0;
-  };
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int m = lane;
-  for (; m + 192 < len_i; m += 256) {
-    s0 += term(m);
-    s1 += term(m + 64);
-    s2 += term(m + 128);
-    s3 += term(m + 192);
-  }
-  for (; m < len_i; m += 64) s0 += term(m);
-  return wave_sum_f64((s0 + s1) + (s2 + s3));
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = wave_sum_f64(sa[q] + sb[q]);
 }
 
 template <int R>
@@ -628,6 +660,10 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
     apply_window<R>(y, a.winmask, lane);
     const long long idx = a.out_ld ? (long long)(i - a.out_row0) * a.out_ld + (j - a.out_col0)
                                    : packed_index(i, j, a.n_events);
+    // candidate slots of both sides first (one ballot per slot, kept in this
+    // wave's LDS buffer, free after the transform), so the transform's
+    // registers are dead during the FP64 evaluation
+    unsigned long long* cmask = reinterpret_cast<unsigned long long*>(wl);  // [side][2R]
     for (int sd = 0; sd < 2; ++sd) {
       if (!(side & (1 << sd))) continue;
       const float sg = sd == 0 ? 1.f : -1.f;
@@ -641,56 +677,76 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) bv = fmaxf(bv, __shfl_xor(bv, o, 64));
       const float cut = bv - a.tie_tol;
-      // candidate slots: one ballot per slot, kept in this wave's LDS buffer
-      // (free after the transform), so the FP64 evaluation below exists once
-      // in the code instead of once per register slot
-      unsigned long long* cmask = reinterpret_cast<unsigned long long*>(wl);
       sfor<0, 2 * R>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         const float v = sg * ((s & 1) ? y[s >> 1].y : y[s >> 1].x);
         const int lg = slot_lag<R>(lane, s);
         const bool inw = lg >= a.lag_lo && lg <= a.lag_hi;
         const unsigned long long m = __ballot(inw && v >= cut && v > 0.f);
-        if (lane == 0) cmask[s] = m;
+        if (lane == 0) cmask[sd * 2 * R + s] = m;
       });
-      wave_lds_fence();
+    }
+    wave_lds_fence();
+    for (int sd = 0; sd < 2; ++sd) {
+      if (!(side & (1 << sd))) continue;
+      const double sg = sd == 0 ? 1.0 : -1.0;
       double best = 0.0;
       int best_lag = 0;
       bool have = false;
+      // the candidates in batches of four, each batch one pass over the traces
+      // per channel (the FP64 sums of the reference's own arithmetic: the
+      // convolution with the caller's reversed row when there is one)
+      int batch[4] = {0, 0, 0, 0};
+      int nb = 0;
+      auto flush = [&]() {
+        if (nb == 0) return;
+        for (int q = nb; q < 4; ++q) batch[q] = batch[0];
+        int shift[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          int t = a.rev_traces ? (batch[q] + a.rev_shift) % P : batch[q];
+          shift[q] = t < 0 ? t + P : t;
+        }
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < a.n_channels; ++c) {
+          const double ni = a.norms[(long long)i * a.n_channels + c];
+          const double nj = a.norms[(long long)j * a.n_channels + c];
+          if (!(ni > 0.0 && nj > 0.0)) continue;
+          const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
+          double sum[4];
+          if (a.rev_traces)
+            corr4_f64(ti, a.tr_len, a.rev_traces + ((long long)j * a.n_channels + c) * a.rev_stride, a.rev_len, P,
+                      shift, true, lane, sum);
+          else
+            corr4_f64(ti, a.tr_len, a.traces + ((long long)j * a.n_channels + c) * a.tr_stride, a.tr_len, P, shift,
+                      false, lane, sum);
+          const double w = ni * nj * (double)a.n_channels;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[q] += sum[q] / w;
+        }
+        for (int q = 0; q < nb; ++q) {
+          const double v = sg * acc[q];
+          if (v > 0.0 && (!have || v > best || (v == best && batch[q] < best_lag))) {
+            best = v;
+            best_lag = batch[q];
+            have = true;
+          }
+        }
+        nb = 0;
+      };
       for (int s = 0; s < 2 * R; ++s) {
-        const unsigned long long ms = cmask[s];
+        const unsigned long long ms = cmask[sd * 2 * R + s];
         unsigned long long mask =
             ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(ms >> 32)) << 32) |
             (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)ms);
         while (mask) {
           const int src = __ffsll((long long)mask) - 1;
           mask &= mask - 1;
-          const int lag = slot_lag<R>(src, s);
-          double acc = 0.0;
-          for (int c = 0; c < a.n_channels; ++c) {
-            const double ni = a.norms[(long long)i * a.n_channels + c];
-            const double nj = a.norms[(long long)j * a.n_channels + c];
-            if (!(ni > 0.0 && nj > 0.0)) continue;
-            const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
-            double cij;
-            if (a.rev_traces) {
-              const float* rj = a.rev_traces + ((long long)j * a.n_channels + c) * a.rev_stride;
-              cij = conv_f64(ti, a.tr_len, rj, a.rev_len, P, lag + a.rev_shift, lane);
-            } else {
-              const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
-              cij = corr_f64(ti, tj, a.tr_len, P, lag, lane);
-            }
-            acc += cij / (ni * nj * (double)a.n_channels);
-          }
-          acc *= (double)sg;
-          if (acc > 0.0 && (!have || acc > best || (acc == best && lag < best_lag))) {
-            best = acc;
-            best_lag = lag;
-            have = true;
-          }
+          batch[nb++] = slot_lag<R>(src, s);
+          if (nb == 4) flush();
         }
       }
-      wave_lds_fence();  // the next side or record rewrites the buffer
+      flush();
       if (lane == 0) {
         const float v = (float)best;
         if (sd == 0) {
@@ -704,6 +760,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
         }
       }
     }
+    wave_lds_fence();  // the next record rewrites the buffer
   }
 }
 

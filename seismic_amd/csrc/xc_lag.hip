@@ -124,23 +124,20 @@ struct StepData {
   u32x4 bh[3], bl[3];
 };
 
+// One K step's operands from this lane's row (A) and column window (B)
+// bases, `off` elements into the plane row (the step's channel and sample).
 template <bool BF16X3>
 __device__ __forceinline__ void load_step(StepData& s, const unsigned short* __restrict__ ahi,
                                           const unsigned short* __restrict__ alo,
                                           const unsigned short* __restrict__ bhi,
-                                          const unsigned short* __restrict__ blo, long long ls,
-                                          int steps_per_channel, int step, int p0, int lb, int h) {
-  const int c = step / steps_per_channel;
-  const int n0 = 16 * (step - c * steps_per_channel);
-  const long long ia = c * ls + p0 + n0 + 8 * h;            // A: 8 samples, 16-byte aligned
-  const long long ib = c * ls + p0 + n0 + 8 * h - lb - 15;  // B: 24 samples, 16-byte aligned
-  s.ah = *reinterpret_cast<const u32x4*>(ahi + ia);
+                                          const unsigned short* __restrict__ blo, int off) {
+  s.ah = *reinterpret_cast<const u32x4*>(ahi + off);
 #pragma unroll
-  for (int q = 0; q < 3; ++q) s.bh[q] = *reinterpret_cast<const u32x4*>(bhi + ib + 8 * q);
+  for (int q = 0; q < 3; ++q) s.bh[q] = *reinterpret_cast<const u32x4*>(bhi + off + 8 * q);
   if constexpr (BF16X3) {
-    s.al = *reinterpret_cast<const u32x4*>(alo + ia);
+    s.al = *reinterpret_cast<const u32x4*>(alo + off);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) s.bl[q] = *reinterpret_cast<const u32x4*>(blo + ib + 8 * q);
+    for (int q = 0; q < 3; ++q) s.bl[q] = *reinterpret_cast<const u32x4*>(blo + off + 8 * q);
   }
 }
 
@@ -183,22 +180,46 @@ __device__ __forceinline__ void mfma_step(f32x16 (&acc)[kLagBlock], const StepDa
 // All K steps (channels x samples) of one block of kLagBlock lags.  Operands
 // are loaded kDepth steps ahead (a ring of kDepth buffers) so one wave per
 // SIMD keeps the MFMA pipe fed: 2 steps for bf16x3 (48 MFMAs per step), 4 for
-// fp16 (16 MFMAs per step, fewer registers per buffer).
+// fp16 (16 MFMAs per step, fewer registers per buffer).  The load cursor (the
+// step's element offset in the plane row) advances by additions only: with
+// one wave per SIMD every scalar instruction takes an issue slot from the
+// MFMAs.
 template <bool BF16X3>
 __device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsigned short* __restrict__ ahi,
                                           const unsigned short* __restrict__ alo,
                                           const unsigned short* __restrict__ bhi,
-                                          const unsigned short* __restrict__ blo, long long ls,
+                                          const unsigned short* __restrict__ blo, int ls,
                                           int n_channels, int l16, int p0, int lb, int h) {
   constexpr int kDepth = BF16X3 ? 2 : 4;
 #pragma unroll
   for (int d = 0; d < kLagBlock; ++d) acc[d] = f32x16{};
-  const int spc = l16 / 16;
-  const int total = n_channels * spc;  // >= 1
+  const int total = n_channels * (l16 / 16);  // >= 1
+  // lane bases: A at sample 8h of the step, B window 15 + lb samples earlier
+  ahi += p0 + 8 * h;
+  bhi += p0 + 8 * h - lb - 15;
+  if constexpr (BF16X3) {
+    alo += p0 + 8 * h;
+    blo += p0 + 8 * h - lb - 15;
+  }
+  // cursor: element offset of the next step to load, samples of its channel
+  // before it, and steps left before it stops at the last one (loads past
+  // the end repeat that step: harmless and in bounds)
+  int ld_off = 0, ld_n = 0, ld_left = total - 1;
+  auto advance = [&]() {
+    if (ld_left > 0) {
+      --ld_left;
+      ld_n += 16;
+      const bool wrap = ld_n == l16;
+      ld_off += wrap ? ls - l16 + 16 : 16;
+      ld_n = wrap ? 0 : ld_n;
+    }
+  };
   StepData buf[kDepth];
 #pragma unroll
-  for (int q = 0; q < kDepth; ++q)
-    load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ls, spc, min(q, total - 1), p0, lb, h);
+  for (int q = 0; q < kDepth; ++q) {
+    load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ld_off);
+    advance();
+  }
   for (int st = 0; st < total; st += kDepth) {
 #pragma unroll
     for (int q = 0; q < kDepth; ++q) {
@@ -213,8 +234,8 @@ __device__ __forceinline__ void lag_block(f32x16 (&acc)[kLagBlock], const unsign
           __builtin_amdgcn_sched_group_barrier(0x002, BF16X3 ? 3 : 5, 0);  // then up to this many VALU
         }
       }
-      // refill (clamped at the end: a repeated in-bounds load is harmless)
-      load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ls, spc, min(st + q + kDepth, total - 1), p0, lb, h);
+      load_step<BF16X3>(buf[q], ahi, alo, bhi, blo, ld_off);
+      advance();
     }
   }
 }
@@ -276,7 +297,7 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
     for (int b = wv; b < a.n_blocks; b += 4) {
       const int lb = a.lag_first + kLagBlock * b;
       f32x16 acc[kLagBlock];
-      lag_block<BF16X3>(acc, ahi, alo, bhi, blo, a.ls, a.n_channels, a.l16, a.p0, lb, h);
+      lag_block<BF16X3>(acc, ahi, alo, bhi, blo, (int)a.ls, a.n_channels, a.l16, a.p0, lb, h);
       // top-3 values (top-2 lags) per pair over the block's lags, in increasing lag order
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
