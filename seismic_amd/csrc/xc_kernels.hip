@@ -325,7 +325,6 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
   // c2r ignores the imaginary parts of the DC and Nyquist bins
   if (lane == 0) y[0].y = 0.f;
 
-  const int src = (64 - lane) & 63;
   const float2* pre = tw + TwLayout<R>::kPre + lane;
   const cf u = CT ? f2c(tw[TwLayoutCT<R>::kPreL + lane]) : cmk(0.f, 0.f);
   // exp(+2 pi i (lane + 64 m) / P)
@@ -334,7 +333,6 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     if constexpr (CT) return twmul<mm, 2 * R, 1>(u);
     else return f2c(pre[mm * 64]);
   };
-  cf saved = cmk(yN, 0.f);  // lane 0: original y[R - m]
   // Z = E + i T D with E = Y + conj(Ym), D = Y - conj(Ym), T = e^{2 pi i k/P}
   auto zform = [](cf a, cf am, cf t) {
     const float ex = a.x + am.x, ey = a.y - am.y;
@@ -342,41 +340,48 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     return cmk(__builtin_fmaf(-t.x, dy, __builtin_fmaf(-t.y, dx, ex)),
                __builtin_fmaf(t.x, dx, __builtin_fmaf(-t.y, dy, ey)));
   };
-  // The mirror values travel by ds_bpermute (lane t <- lane 64 - t); those of
-  // iteration m + kAhead are requested before iteration m is computed, so the
-  // LDS round trips overlap the arithmetic instead of ending each iteration.
-  constexpr int kAhead = 2;
-  cf sh_b[kAhead], sh_a[kAhead];  // shuffled y[R - 1 - m], y[m] of iterations m .. m + kAhead - 1
-  auto request = [&](auto mc, cf& rb, cf& ra) {
-    constexpr int m = decltype(mc)::value;
-    rb.x = __shfl(y[R - 1 - m].x, src, 64);
-    rb.y = __shfl(y[R - 1 - m].y, src, 64);
-    ra.x = __shfl(y[m].x, src, 64);
-    ra.y = __shfl(y[m].y, src, 64);
-  };
-  sfor<0, (kAhead < R / 2 ? kAhead : R / 2)>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    request(qc, sh_b[q], sh_a[q]);
+  // The mirrored bins go through this wave's LDS buffer: bin k sits at lane
+  // t = k % 64, register k / 64, so written as rows of 64 the bins of a few
+  // consecutive registers form a linear range, and every lane reads its
+  // mirror M - k at one address formula — lane 0, whose mirrors are its own
+  // register R - m (the Nyquist bin for m = 0), needs no special case.  In
+  // chunks of Q low registers m (and their partners R - 1 - m): the Q + 1
+  // registers holding the low ones' mirrors, then the Q + 1 holding the high
+  // ones', 2 (Q + 1) 64-lane rows in the 9-row buffer, before any of the
+  // chunk's values change.  The top row of the first set is the register the
+  // previous chunk turned into Z first, so its original is kept.
+  constexpr int Q = 4;
+  static_assert((R / 2) % Q == 0 && Q + 1 <= kWaveLds / 64, "mirror chunks");
+  cf top = cmk(yN, 0.f);  // original y[R - Q c] (the Nyquist bin for c = 0)
+  sfor<0, R / (2 * Q)>([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    constexpr int a0 = R - Q - Q * c;  // first register of the low ones' mirror set
+    constexpr int b0 = Q * c;          // first register of the high ones' mirror set
+    cf am[Q], bm[Q];
+    wave_lds_fence();  // earlier reads of the buffer come first
+#pragma unroll
+    for (int r = 0; r < Q; ++r) wl[64 * r + lane] = c2f(y[a0 + r]);
+    wl[64 * Q + lane] = c2f(top);
+    wave_lds_fence();
+    // low m = b0 + q: mirror M - lane - 64 m is row Q - q, lane offset -lane
+#pragma unroll
+    for (int q = 0; q < Q; ++q) am[q] = f2c(wl[64 * (Q - q) - lane]);
+    wave_lds_fence();
+#pragma unroll
+    for (int r = 0; r <= Q; ++r) wl[64 * r + lane] = c2f(y[b0 + r]);
+    wave_lds_fence();
+    // high R - 1 - m: mirror 64 (m + 1) - lane is row q + 1
+#pragma unroll
+    for (int q = 0; q < Q; ++q) bm[q] = f2c(wl[64 * (q + 1) - lane]);
+    top = y[a0];  // the next chunk's top row, about to change below
+    sfor<0, Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int m = b0 + q, mh = R - 1 - m;
+      y[m] = zform(y[m], am[q], pre_tw(std::integral_constant<int, m>{}));
+      y[mh] = zform(y[mh], bm[q], pre_tw(std::integral_constant<int, mh>{}));
+    });
   });
-  sfor<0, R / 2>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    constexpr int mh = R - 1 - m;
-    constexpr int slot = m % kAhead;
-    cf a = y[m], b = y[mh];
-    cf am = sh_b[slot], bm = sh_a[slot];
-    if constexpr (m + kAhead < R / 2) request(std::integral_constant<int, m + kAhead>{}, sh_b[slot], sh_a[slot]);
-    {
-      const cf nb = (m + 1 == mh) ? b : y[m + 1];
-      const bool l0 = lane == 0;
-      am.x = l0 ? saved.x : am.x;
-      am.y = l0 ? saved.y : am.y;
-      bm.x = l0 ? nb.x : bm.x;
-      bm.y = l0 ? nb.y : bm.y;
-    }
-    saved = b;
-    y[m] = zform(a, am, pre_tw(std::integral_constant<int, m>{}));
-    y[mh] = zform(b, bm, pre_tw(std::integral_constant<int, mh>{}));
-  });
+  wave_lds_fence();  // the FFT's transposes reuse the buffer
   wave_fft<R, 1, CT>(y, tw, wl, lane);
 }
 
