@@ -384,33 +384,36 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
 // ----------------------------------------------------------------------------
 
 // FP64 sums sum_n ti[n] tj[n - lag_q] over the overlap of the two traces in
-// [0, L), for four lags at once (callers repeat a lag to fill unused slots):
-// each ti sample is read once for all of them, the loads of all four are
+// [0, L), for NQ lags at once (callers repeat a lag to fill unused slots):
+// each ti sample is read once for all of them, the loads of all NQ are
 // issued together (no per-lag branches), and two sets of partial sums per
 // lane keep the FP64 FMA chains short.  The results are wave-uniform.
-__device__ __forceinline__ void lin_corr4_f64(const float* __restrict__ ti, const float* __restrict__ tj, int L,
-                                              const int (&lag)[4], int lane, double (&out)[4]) {
-  double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
-  auto step = [&](int n, double (&acc)[4]) {
+template <int NQ>
+__device__ __forceinline__ void lin_corr_f64(const float* __restrict__ ti, const float* __restrict__ tj, int L,
+                                             const int (&lag)[4], int lane, double (&out)[4]) {
+  double sa[NQ], sb[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) sa[q] = sb[q] = 0.0;
+  auto step = [&](int n, double (&acc)[NQ]) {
     const bool in = n < L;
     const double a = (double)ti[in ? n : 0];
-    float b[4];
-    bool ok[4];
+    float b[NQ];
+    bool ok[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const int u = n - lag[q];
       ok[q] = in && u >= 0 && u < L;
       b[q] = tj[ok[q] ? u : 0];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += ok[q] ? a * (double)b[q] : 0.0;
+    for (int q = 0; q < NQ; ++q) acc[q] += ok[q] ? a * (double)b[q] : 0.0;
   };
   for (int n = lane; n < L; n += 128) {
     step(n, sa);
     step(n + 64, sb);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < NQ; ++q) {
     double v = sa[q] + sb[q];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -517,8 +520,10 @@ __global__ __launch_bounds__(64) void k_lag_resolve(LagResolveArgs a) {
         if (!(ni > 0.0 && nj > 0.0)) continue;
         const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
         const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
-        double sum[4];
-        lin_corr4_f64(ti, tj, a.L, lag, lane, sum);
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        // one side's two lags (the usual record) in a two-lag pass
+        if (nq <= 2) lin_corr_f64<2>(ti, tj, a.L, lag, lane, sum);
+        else lin_corr_f64<4>(ti, tj, a.L, lag, lane, sum);
         const double w = ni * nj * (double)a.n_channels;
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] += sum[q] / w;
