@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
 // sums over the time-domain traces and keeps the largest (smallest) with
 // ties to the smaller lag; starting state (0, lag 0) as in the reference scan.
 //
-// FP64 sums for four lags at once (callers repeat one to fill unused slots),
+// FP64 sums for NQ lags at once (callers repeat one to fill unused slots),
 // as shifts in [0, P):
 //   conv = false: sum_n ti[n] tj[(n - shift_q) mod P], the circular
 //                 correlation at lag shift_q;
@@ -604,34 +604,37 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
 //                 own sum for reversed rows (Correlation_in_OMP/
 //                 correlation_c.cpp:270, :63, :102-103): its lag-l value is
 //                 corr_t[L - 1 + l], this sum at shift_q = L - 1 + l.
-// Each ti sample is read once for all four, their loads are issued together,
+// Each ti sample is read once for all NQ, their loads are issued together,
 // and two sets of partial sums per lane keep the FP64 FMA chains short.  The
 // results are wave-uniform.
-__device__ __forceinline__ void corr4_f64(const float* __restrict__ ti, int len_i, const float* __restrict__ tj,
+template <int NQ>
+__device__ __forceinline__ void corrn_f64(const float* __restrict__ ti, int len_i, const float* __restrict__ tj,
                                           int len_j, int P, const int (&shift)[4], bool conv, int lane,
                                           double (&out)[4]) {
-  double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0};
-  auto step = [&](int n, double (&acc)[4]) {
+  double sa[NQ], sb[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) sa[q] = sb[q] = 0.0;
+  auto step = [&](int n, double (&acc)[NQ]) {
     const bool in = n < len_i;
     const double x = (double)ti[in ? n : 0];
-    float b[4];
-    bool ok[4];
+    float b[NQ];
+    bool ok[NQ];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       int u = conv ? shift[q] - n : n - shift[q];  // n, shift in [0, P): one wrap suffices
       u += u < 0 ? P : 0;
       ok[q] = in && u < len_j;
       b[q] = tj[ok[q] ? u : 0];  // always in bounds
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += ok[q] ? x * (double)b[q] : 0.0;
+    for (int q = 0; q < NQ; ++q) acc[q] += ok[q] ? x * (double)b[q] : 0.0;
   };
   for (int n = lane; n < len_i; n += 128) {
     step(n, sa);
     step(n + 64, sb);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) out[q] = wave_sum_f64(sa[q] + sb[q]);
+  for (int q = 0; q < NQ; ++q) out[q] = wave_sum_f64(sa[q] + sb[q]);
 }
 
 template <int R>
@@ -718,13 +721,17 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
           const double nj = a.norms[(long long)j * a.n_channels + c];
           if (!(ni > 0.0 && nj > 0.0)) continue;
           const float* ti = a.traces + ((long long)i * a.n_channels + c) * a.tr_stride;
-          double sum[4];
-          if (a.rev_traces)
-            corr4_f64(ti, a.tr_len, a.rev_traces + ((long long)j * a.n_channels + c) * a.rev_stride, a.rev_len, P,
-                      shift, true, lane, sum);
-          else
-            corr4_f64(ti, a.tr_len, a.traces + ((long long)j * a.n_channels + c) * a.tr_stride, a.tr_len, P, shift,
-                      false, lane, sum);
+          double sum[4] = {0.0, 0.0, 0.0, 0.0};
+          // a side's usual two candidates in a two-lag pass
+          if (a.rev_traces) {
+            const float* rj = a.rev_traces + ((long long)j * a.n_channels + c) * a.rev_stride;
+            if (nb <= 2) corrn_f64<2>(ti, a.tr_len, rj, a.rev_len, P, shift, true, lane, sum);
+            else corrn_f64<4>(ti, a.tr_len, rj, a.rev_len, P, shift, true, lane, sum);
+          } else {
+            const float* tj = a.traces + ((long long)j * a.n_channels + c) * a.tr_stride;
+            if (nb <= 2) corrn_f64<2>(ti, a.tr_len, tj, a.tr_len, P, shift, false, lane, sum);
+            else corrn_f64<4>(ti, a.tr_len, tj, a.tr_len, P, shift, false, lane, sum);
+          }
           const double w = ni * nj * (double)a.n_channels;
 #pragma unroll
           for (int q = 0; q < 4; ++q) acc[q] += sum[q] / w;
