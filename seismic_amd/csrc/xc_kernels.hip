@@ -138,24 +138,6 @@ __device__ __forceinline__ void forward_row(const ForwardArgs& a, int row, const
     if (lane == 0 && !a.reversed) a.norms[row] = 0.0;
     return;
   }
-  // mean and energy of the first L samples, FP64, straight from memory (a
-  // reversed row takes the energy of its forward row instead)
-  double mean = 0.0, E = 0.0;
-  if (a.reversed) {
-    const double nr = a.norms_in[row];
-    E = nr * nr;
-  } else {
-    double s1 = 0.0;
-    for (int n = lane; n < L; n += 64) s1 += (double)(n < in_len ? x[n] : 0.f);
-    mean = a.demean ? wave_sum_f64(s1) / (double)L : 0.0;
-    double s2 = 0.0;
-    for (int n = lane; n < L; n += 64) {
-      const float v = n < in_len ? x[n] : 0.f;
-      const float u = a.demean ? (float)((double)v - mean) : v;
-      s2 += (double)u * (double)u;
-    }
-    E = wave_sum_f64(s2);
-  }
   // d[m] = x[2(t+64m)] + i x[2(t+64m)+1]; clamped loads + selects, no branches
   cf d[R];
   const int last = in_len > 0 ? in_len - 1 : 0;
@@ -166,14 +148,33 @@ __device__ __forceinline__ void forward_row(const ForwardArgs& a, int row, const
     d[m].x = n < in_len ? e0 : 0.f;
     d[m].y = n + 1 < in_len ? e1 : 0.f;
   }
-  if (a.demean) {
+  // mean (from memory) and energy (from the registers) of the first L
+  // samples in FP64 (a reversed row takes the energy of its forward row)
+  double mean = 0.0, E = 0.0;
+  if (a.reversed) {
+    const double nr = a.norms_in[row];
+    E = nr * nr;
+  } else {
+    if (a.demean) {
+      double s1 = 0.0;
+      for (int n = lane; n < L; n += 64) s1 += (double)(n < in_len ? x[n] : 0.f);
+      mean = wave_sum_f64(s1) / (double)L;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int n = 2 * (lane + 64 * m);
+        const float u0 = (float)((double)d[m].x - mean), u1 = (float)((double)d[m].y - mean);
+        d[m].x = n < L ? u0 : d[m].x;
+        d[m].y = n + 1 < L ? u1 : d[m].y;
+      }
+    }
+    double s2 = 0.0;
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int n = 2 * (lane + 64 * m);
-      const float u0 = (float)((double)d[m].x - mean), u1 = (float)((double)d[m].y - mean);
-      d[m].x = n < L ? u0 : d[m].x;
-      d[m].y = n + 1 < L ? u1 : d[m].y;
+      const double u0 = n < L ? (double)d[m].x : 0.0, u1 = n + 1 < L ? (double)d[m].y : 0.0;
+      s2 += u0 * u0 + u1 * u1;
     }
+    E = wave_sum_f64(s2);
   }
   if (a.out_traces) {
     float* o = a.out_traces + (long long)row * a.out_stride;
