@@ -562,9 +562,10 @@ __global__ __launch_bounds__(kWaves * 64, (R > 32 ? 1 : 2)) void k_pairs(PairArg
     for (int j = (dense ? jb : max(jb, i)) + wv; j < je; j += kWaves) {
       cf y[R];
       const float2* xj = cols + (long long)j * a.n_channels * a.spec_stride;
-      // smaller load groups at P >= 6144 where registers are short: rows from
-      // global memory (twice the loads in flight) and the 12-wave budget
-      pair_inverse<R, !XI_LDS, CT, (R >= 48 && (!XI_LDS || kWaves == 12)) ? 16 : 32>(
+      // smaller load groups where registers are short: the 16-wave budget
+      // (128 VGPRs), and at P >= 6144 rows from global memory (twice the
+      // loads in flight) and the 12-wave budget
+      pair_inverse<R, !XI_LDS, CT, (kWaves == 16 || (R >= 48 && (!XI_LDS || kWaves == 12))) ? 16 : 32>(
           y, XI_LDS ? xi_lds : src, xs, xj, a.spec_stride, a.n_channels, twl, wl, lane);
       if constexpr (!FULL) apply_window<R>(y, a.winmask, lane);
       Best pos, neg;
@@ -808,12 +809,16 @@ template <int R, bool FULL>
 static hipError_t launch_pairs_rf(const PairArgs& a, int grid, hipStream_t st) {
   // Compact twiddle tables throughout (fewer LDS reads per pair for a few
   // more multiplies: measured 1-1.3% faster at P <= 4096, 22% at P = 6144).
-  // 12 waves (3 per SIMD) when the staged row spectra fit the 160 KiB of a
-  // CU, else 8; rows from global memory when neither fits.  P = 8192 always
-  // runs 8 waves: its 128 spectrum registers per lane need the 256-VGPR
-  // budget.
+  // Up to P = 4096, 16 waves (4 per SIMD; 125 VGPRs at P = 4096 with
+  // channel loads in groups of 16 bins) when the staged row spectra fit the
+  // 160 KiB of a CU; else 12 waves (3 per SIMD) when they fit, else 8; rows
+  // from global memory when none fits.  P = 8192 always runs 8 waves: its 128
+  // spectrum registers per lane need the 256-VGPR budget.
   constexpr size_t kLds = 160 * 1024;
   const int C = a.n_channels;
+  if constexpr (R <= 32) {
+    if (pair_lds_bytes(R, C, 16, true) <= kLds) return launch_pairs_rfw<R, FULL, 16, true, true>(a, grid, st);
+  }
   if constexpr (R <= 48) {
     if (pair_lds_bytes(R, C, 12, true) <= kLds) return launch_pairs_rfw<R, FULL, 12, true, true>(a, grid, st);
   }
