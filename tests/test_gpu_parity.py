@@ -286,13 +286,39 @@ def test_extended_p8192_channel_counts(oracle, C, full):
 def test_extended_mixed_radix_periods(oracle, P, C, full):
     """Periods 3 * 2^k and 5 * 2^k (a radix-3 or radix-5 stage in the
     register DFT), traces longer than P/2 so the circular wrap matters, over
-    the kernel variants: 12 waves (P=3072 up to 8 channels, P=5120 and 6144
-    up to 3), 8 waves (P=5120 with 5, P=6144 with 4 channels), rows from
-    global memory (P=5120 with 8, P=6144 with 5 channels)."""
+    the kernel variants: 16 waves (P=3072 with 1 and 3 channels), 12 waves
+    (P=3072 with 8, P=5120 with 1 and 3, P=6144 with 1, 3 and 4), 8 waves
+    (P=5120 with 5), rows from global memory (P=5120 with 8, P=6144 with
+    5 channels)."""
     from seismic_amd import engine
     n, L = 12, P - 200
     ev, _ = synth.make_events(n, C, L, seed=90 + C + P // 1024)
     lo, hi = (-P // 2 + 1, P // 2) if full else (-1000, 1400)
+    print(f"case P={P} C={C} L={L} window [{lo}, {hi}]", flush=True)
+    vp, lp, vn, ln = engine.allpairs_host(ev, P, lo, hi)
+    pi, pj = oracle.packed_pairs(n)
+    ref = oracle.pairs_f64(ev, pi, pj, lo, hi, P=P, circular=True)
+    dp, dn = np.abs(vp - ref["pos"]).max(), np.abs(vn - ref["neg"]).max()
+    print(f"P={P} C={C}: max |dcoef| pos {dp:.3e} neg {dn:.3e}")
+    assert dp <= COEF_TOL and dn <= COEF_TOL
+    sure_p = ref["gap_pos"] >= GPU_EXACT_GAP
+    sure_n = ref["gap_neg"] >= GPU_EXACT_GAP
+    assert np.array_equal(lp[sure_p], ref["lag_pos"][sure_p])
+    assert np.array_equal(ln[sure_n], ref["lag_neg"][sure_n])
+
+
+@pytest.mark.parametrize("P,C,full", [(1024, 8, True), (2048, 8, False), (4096, 5, True), (4096, 5, False),
+                                      (4096, 6, False), (4096, 7, True), (4096, 8, False)])
+def test_extended_pow2_kernel_variants(oracle, P, C, full):
+    """Power-of-two periods over the pair kernel's variants, chosen by what
+    the staged row spectra leave of the 160 KiB of LDS: 16 waves (P <= 2048
+    up to 8 channels, P=4096 up to 5), 12 waves (P=4096 with 6), 8 waves
+    (P=4096 with 7), rows from global memory (P=4096 with 8); traces longer
+    than P/2 so the circular wrap matters."""
+    from seismic_amd import engine
+    n, L = 12, P - 200
+    ev, _ = synth.make_events(n, C, L, seed=70 + C + P // 1024)
+    lo, hi = (-P // 2 + 1, P // 2) if full else (-300, 500)
     print(f"case P={P} C={C} L={L} window [{lo}, {hi}]", flush=True)
     vp, lp, vn, ln = engine.allpairs_host(ev, P, lo, hi)
     pi, pj = oracle.packed_pairs(n)
