@@ -321,8 +321,16 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     const float2 aN = ai[M], bN = xj[c * xj_stride + M];
     yN = kFirst ? __builtin_fmaf(aN.x, bN.x, aN.y * bN.y) : __builtin_fmaf(aN.x, bN.x, __builtin_fmaf(aN.y, bN.y, yN));
   };
+  // Wave priority by phase (s_setprio; the arbiter issues from the waves of
+  // a SIMD in priority order): 2 while the column spectra are loaded and
+  // multiplied, 1 for the packing, 0 for the transform and the scan.  A wave
+  // starting a pair then gets its loads out and its products done first,
+  // which keeps the SIMD's waves staggered over the phases instead of all
+  // waiting on memory or LDS at once (pair kernel 6% faster at P = 4096).
+  __builtin_amdgcn_s_setprio(2);
   channel(0, std::true_type{});
   for (int c = 1; c < nch; ++c) channel(c, std::false_type{});
+  __builtin_amdgcn_s_setprio(1);
   // c2r ignores the imaginary parts of the DC and Nyquist bins
   if (lane == 0) y[0].y = 0.f;
 
@@ -383,6 +391,7 @@ __device__ __forceinline__ void pair_inverse(cf (&y)[R], const float2* __restric
     });
   });
   wave_lds_fence();  // the FFT's transposes reuse the buffer
+  __builtin_amdgcn_s_setprio(0);
   wave_fft<R, 1, CT>(y, tw, wl, lane);
 }
 
