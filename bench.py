@@ -495,7 +495,7 @@ def cpu_baseline(ev: np.ndarray, P: int, S: int, sample_events: int, threads: in
                       f"restatement of Correlation_in_OMP with channel stacking, own FP32 FFT "
                       f"(FFTW3f is not installed), OpenMP dynamic schedule, chunk 4",
             "threads_rule": "every CPU this process may use (sched_getaffinity), capped by OMP_NUM_THREADS "
-                            "when set: a one-GPU box's CPU share",
+                            "when set (a host shared by several GPUs gives each its share)",
             **host}
 
 
