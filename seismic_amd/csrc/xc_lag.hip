@@ -385,32 +385,43 @@ __global__ __launch_bounds__(256, 1) void k_lag_pairs(LagPairArgs a) {
 
 // FP64 sums sum_n ti[n] tj[n - lag_q] over the overlap of the two traces in
 // [0, L), for NQ lags at once (callers repeat a lag to fill unused slots):
-// each ti sample is read once for all of them, the loads of all NQ are
-// issued together (no per-lag branches), and two sets of partial sums per
-// lane keep the FP64 FMA chains short.  The results are wave-uniform.
+// each ti sample is read once for all of them, the loads of 8 steps of 64
+// samples and all NQ lags are issued together before any is used (no
+// per-lag branches; one memory round trip per 512 samples: fp16 resolver
+// 6.1 -> 5.6 ms at 753k records), and two sets of partial sums per lane
+// keep the FP64 FMA chains short.  The results are wave-uniform.
 template <int NQ>
 __device__ __forceinline__ void lin_corr_f64(const float* __restrict__ ti, const float* __restrict__ tj, int L,
                                              const int (&lag)[4], int lane, double (&out)[4]) {
+  constexpr int U = 8;  // 64-sample steps per turn
   double sa[NQ], sb[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) sa[q] = sb[q] = 0.0;
-  auto step = [&](int n, double (&acc)[NQ]) {
-    const bool in = n < L;
-    const double a = (double)ti[in ? n : 0];
-    float b[NQ];
-    bool ok[NQ];
+  for (int n0 = lane; n0 < L; n0 += U * 64) {
+    float av[U], bv[U][NQ];
+    bool ok[U][NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int u = n - lag[q];
-      ok[q] = in && u >= 0 && u < L;
-      b[q] = tj[ok[q] ? u : 0];
+    for (int k = 0; k < U; ++k) {
+      const int n = n0 + 64 * k;
+      const bool in = n < L;
+      av[k] = ti[in ? n : 0];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int u = n - lag[q];
+        ok[k][q] = in && u >= 0 && u < L;
+        bv[k][q] = tj[ok[k][q] ? u : 0];
+      }
     }
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] += ok[q] ? a * (double)b[q] : 0.0;
-  };
-  for (int n = lane; n < L; n += 128) {
-    step(n, sa);
-    step(n + 64, sb);
+    for (int k = 0; k < U; ++k) {
+      const double a = (double)av[k];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double& acc = (k & 1) ? sb[q] : sa[q];
+        acc += ok[k][q] ? a * (double)bv[k][q] : 0.0;
+      }
+    }
   }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
