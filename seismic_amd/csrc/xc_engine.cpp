@@ -121,7 +121,7 @@ void plan_free(sxc_plan* p) {
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->col_spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
                   p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles,
-                  p->fnorms, p->wide, p->wide_count};
+                  p->fnorms, p->wide, p->wide_count, p->band_work, p->band_count};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -449,6 +449,10 @@ struct HostJob {
   sxc_plan* p = nullptr;
   float *in = nullptr, *rev = nullptr, *vp = nullptr, *vn = nullptr;
   int *lp = nullptr, *ln = nullptr, *hist = nullptr;
+  // banded runs: compute and copy streams (non-blocking, default priority)
+  // and one event per band, created on first use
+  hipStream_t cs = nullptr, xs = nullptr;
+  std::vector<hipEvent_t> band_ev;
 };
 
 // Device bytes a plan holds (its large buffers; tables and counters aside).
@@ -457,7 +461,8 @@ static size_t plan_device_bytes(const sxc_plan* p) {
   size_t b = rows * (size_t)p->spec_stride * sizeof(float2) * ((p->spec ? 1 : 0) + (p->col_spec ? 1 : 0));
   b += rows * sizeof(double) * ((p->norms ? 1 : 0) + (p->fnorms ? 1 : 0));
   if (p->traces) b += rows * (size_t)p->P * sizeof(float);
-  b += (size_t)p->n_work * sizeof(int2) + (size_t)p->tie_cap * sizeof(int4) * (p->lag ? 3 : 1);
+  b += (size_t)p->n_work * sizeof(int2) * (p->band_work ? 2 : 1) +
+       (size_t)p->tie_cap * sizeof(int4) * (p->lag ? 3 : 1);
   if (p->lag) b += rows * (size_t)p->ls * 2 * (p->lo ? 2 : 1) + rows * (size_t)p->d.trace_len * sizeof(float);
   return b;
 }
@@ -467,6 +472,9 @@ static void job_free(HostJob* j) {
   void* bufs[] = {j->in, j->rev, j->vp, j->vn, j->lp, j->ln, j->hist};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (hipEvent_t e : j->band_ev) (void)hipEventDestroy(e);
+  if (j->cs) (void)hipStreamDestroy(j->cs);
+  if (j->xs) (void)hipStreamDestroy(j->xs);
   sxc_plan_destroy(j->p);
   delete j;
 }
@@ -547,6 +555,86 @@ static void job_release(HostJob* j) {
   for (HostJob* x : drop) job_free(x);
 }
 
+static int check_execute_args(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
+                              float* vp, int* lp, float* vn, int* ln) {
+  if (!p || !d_traces || !vp || !lp || !vn || !ln) {
+    set_error("null argument");
+    return SXC_ERR_INVALID;
+  }
+  if (!p->lag && (in_len < 1 || in_len > p->P || trace_stride < in_len)) {
+    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  if (p->lag && (in_len > trace_stride || in_len < 1)) {
+    set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  return SXC_OK;
+}
+
+// The reversed-row checks of an FFT-form execute, and its column-spectrum
+// buffer (allocated on the first reversed-row execute of the plan).
+static int prepare_execute_reversed(sxc_plan* p, const float* d_traces, const float* d_reversed,
+                                    long long trace_stride, int in_len) {
+  if (!d_traces || !d_reversed) { set_error("null argument"); return SXC_ERR_INVALID; }
+  if (p->lag) { set_error("reversed rows are an FFT-form execute (the lag forms correlate the traces directly)"); return SXC_ERR_INVALID; }
+  if (p->d.demean) { set_error("reversed rows cannot be demeaned consistently: create the plan with demean = 0"); return SXC_ERR_INVALID; }
+  if (in_len < 1 || in_len > p->P || trace_stride < in_len) {
+    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
+    return SXC_ERR_INVALID;
+  }
+  if (!p->col_spec)
+    return dev_alloc(&p->col_spec, (size_t)((long long)p->d.n_events * p->d.n_channels * p->spec_stride),
+                     "reversed-row spectra");
+  return SXC_OK;
+}
+
+// Host-buffer FFT-form runs of at least this many events go in row bands:
+// the pair kernel and the resolver of band b + 1 run while band b's packed
+// outputs are copied to the host (the copies of the last band alone are not
+// hidden).
+static constexpr int kBands = 4;
+static constexpr int kBandMinEvents = 1024;
+
+// Banded execute and output copies of a host-buffer job (inputs on the device).
+static int run_banded(HostJob* j, bool reversed, long long trace_stride, int in_len, float* vals_pos,
+                      int* lags_pos, float* vals_neg, int* lags_neg) {
+  sxc_plan* p = j->p;
+  int rc;
+  if ((rc = fft_plan_bands(p, kBands))) return rc;
+  if (!j->cs) HIP_CHECK(hipStreamCreateWithFlags(&j->cs, hipStreamNonBlocking));
+  if (!j->xs) HIP_CHECK(hipStreamCreateWithFlags(&j->xs, hipStreamNonBlocking));
+  while ((int)j->band_ev.size() < kBands) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    j->band_ev.push_back(e);
+  }
+  p->last_stream = j->cs;
+  for (int b = 0; b < kBands; ++b) {
+    if ((rc = fft_plan_execute_band(p, b, j->in, reversed ? j->rev : nullptr, trace_stride, in_len, j->vp, j->lp,
+                                    j->vn, j->ln, j->cs)))
+      return rc;
+    HIP_CHECK(hipEventRecord(j->band_ev[b], j->cs));
+  }
+  const long long n = p->d.n_events;
+  auto packed = [n](long long r) { return r * n - r * (r - 1) / 2; };  // first index of row r
+  for (int b = 0; b < kBands && !rc; ++b) {
+    const size_t o = (size_t)packed(p->band_row[b]), c = (size_t)packed(p->band_row[b + 1]) - o;
+    HIP_CHECK(hipStreamWaitEvent(j->xs, j->band_ev[b], 0));
+    const HostSeg segs[4] = {{vals_pos + o, j->vp + o, c * sizeof(float)},
+                             {lags_pos + o, j->lp + o, c * sizeof(int)},
+                             {vals_neg + o, j->vn + o, c * sizeof(float)},
+                             {lags_neg + o, j->ln + o, c * sizeof(int)}};
+    rc = copy_segs_to_host(segs, 4, j->xs);
+  }
+  hipError_t es = hipStreamSynchronize(j->cs);
+  if (es != hipSuccess) {
+    set_error("device execution failed: %s", hipGetErrorString(es));
+    return SXC_ERR_HIP;
+  }
+  return rc;
+}
+
 // Host-memory run: copy traces (and reversed rows) in, execute, check, copy
 // the packed outputs back.  The outputs are written even when near ties
 // overflowed (the error is still returned).
@@ -564,8 +652,26 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
   const long long rows = (long long)desc->n_events * desc->n_channels;
   const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
   const size_t np = (size_t)j->p->n_pairs;
-  if ((rc = copy_to_device(j->in, traces, in_bytes, nullptr)) ||
-      (reversed && (rc = copy_to_device(j->rev, reversed, in_bytes, nullptr)))) {
+  const bool banded = !j->p->lag && desc->n_events >= kBandMinEvents;
+  const HostSeg in_segs[2] = {{(void*)traces, j->in, in_bytes}, {(void*)reversed, j->rev, in_bytes}};
+  if (banded) {
+    // the checks and lazy set-up of sxc_plan_execute(_reversed), then the bands
+    rc = reversed ? prepare_execute_reversed(j->p, j->in, j->rev, trace_stride, in_len)
+                  : check_execute_args(j->p, j->in, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln);
+    if (!rc && !j->cs) {
+      if (hipStreamCreateWithFlags(&j->cs, hipStreamNonBlocking) != hipSuccess) {
+        set_error("creating a stream failed");
+        rc = SXC_ERR_HIP;
+      }
+    }
+    if (!rc) rc = copy_segs_to_device(in_segs, reversed ? 2 : 1, j->cs);
+    if (!rc) rc = run_banded(j, reversed != nullptr, trace_stride, in_len, vals_pos, lags_pos, vals_neg, lags_neg);
+    const int tie_rc = rc == SXC_OK ? check_tie_overflow(j->p) : SXC_OK;
+    if (rc == SXC_ERR_HIP) job_free(j);  // a failed device may have left the job's state unusable
+    else job_release(j);
+    return rc ? rc : tie_rc;
+  }
+  if ((rc = copy_segs_to_device(in_segs, reversed ? 2 : 1, nullptr))) {
     job_release(j);
     return rc;
   }
@@ -582,11 +688,11 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
   int tie_rc = SXC_OK;
   if (rc == SXC_OK) tie_rc = check_tie_overflow(j->p);
   if (rc == SXC_OK) {
-    int crc = copy_to_host(vals_pos, j->vp, np * sizeof(float), nullptr);
-    if (!crc) crc = copy_to_host(lags_pos, j->lp, np * sizeof(int), nullptr);
-    if (!crc) crc = copy_to_host(vals_neg, j->vn, np * sizeof(float), nullptr);
-    if (!crc) crc = copy_to_host(lags_neg, j->ln, np * sizeof(int), nullptr);
-    if (crc) rc = crc;
+    const HostSeg out_segs[4] = {{vals_pos, j->vp, np * sizeof(float)},
+                                 {lags_pos, j->lp, np * sizeof(int)},
+                                 {vals_neg, j->vn, np * sizeof(float)},
+                                 {lags_neg, j->ln, np * sizeof(int)}};
+    rc = copy_segs_to_host(out_segs, 4, nullptr);
   }
   if (rc == SXC_ERR_HIP) {
     job_free(j);  // a failed device may have left the job's state unusable
@@ -668,18 +774,20 @@ int sxc_plan_destroy(sxc_plan* plan) {
 // (Correlation_in_OMP/correlation_c.cpp:43-45, :270) -- their spectra,
 // rotated by L - 1, become the column spectra, so lag l reads the reference's
 // corr_t[L - 1 + l] whatever length each event really has.
-int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride,
-                          int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
-                          hipStream_t st) {
-  const sxc_plan_desc& d = p->d;
-  const int rows = d.n_events * d.n_channels;
+namespace {
+using namespace sxc;
 
+// The three stages of an FFT-form execute, asynchronous on st.
+// Forward (and reversed-row) R2C of every row, and the near-tie counter reset.
+int fft_stage_forward(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride, int in_len,
+                      hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
   ForwardArgs fa{};
   fa.in = d_traces;
   fa.in_stride = trace_stride;
   fa.in_len = in_len;
   fa.L = d.trace_len;
-  fa.n_rows = rows;
+  fa.n_rows = d.n_events * d.n_channels;
   fa.n_channels = d.n_channels;
   fa.demean = d.demean;
   fa.norms = p->norms;
@@ -689,9 +797,6 @@ int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev
   fa.spec_stride = p->spec_stride;
   fa.tw = p->fwd2;
   fa.post_tw = p->post;
-  hipEvent_t* ev = nullptr;
-  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
-  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
   HIP_CHECK(launch_forward(p->R, fa, st));
   if (d_rev) {
     ForwardArgs ra = fa;
@@ -706,16 +811,21 @@ int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev
     HIP_CHECK(launch_forward(p->R, ra, st));
   }
   HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, st));
-  if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
+  return SXC_OK;
+}
 
+// The pair kernel over work items [work, work + n_work).
+int fft_stage_pairs(sxc_plan* p, bool reversed, const int2* work, int n_work, float* d_vals_pos, int* d_lags_pos,
+                    float* d_vals_neg, int* d_lags_neg, hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
   PairArgs pa{};
   pa.spec = p->spec;
-  pa.col_spec = d_rev ? p->col_spec : nullptr;
+  pa.col_spec = reversed ? p->col_spec : nullptr;
   pa.spec_stride = p->spec_stride;
   pa.n_events = d.n_events;
   pa.n_channels = d.n_channels;
-  pa.work = p->work;
-  pa.n_work = p->n_work;
+  pa.work = work;
+  pa.n_work = n_work;
   pa.chunk = p->chunk;
   pa.lag_lo = d.lag_lo;
   pa.lag_hi = d.lag_hi;
@@ -731,10 +841,16 @@ int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev
   pa.tie_count = p->tie_count;
   pa.tie_cap = p->tie_cap;
   // a multiple of 8 (one share per XCD) whenever there are 8 items or more
-  const int grid = p->n_work >= 4096 ? 4096 : (p->n_work >= 8 ? p->n_work / 8 * 8 : p->n_work);
-  HIP_CHECK(launch_pairs(p->R, p->full, pa, grid, st));
-  if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
+  const int grid = n_work >= 4096 ? 4096 : (n_work >= 8 ? n_work / 8 * 8 : n_work);
+  if (grid > 0) HIP_CHECK(launch_pairs(p->R, p->full, pa, grid, st));
+  return SXC_OK;
+}
 
+// The FP64 resolver over the near-tie records from *tie_begin (null: all).
+int fft_stage_resolve(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride, int in_len,
+                      const unsigned* tie_begin, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg,
+                      int* d_lags_neg, hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
   ResolveArgs ra{};
   ra.spec = p->spec;
   ra.spec_stride = p->spec_stride;
@@ -769,27 +885,93 @@ int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev
   ra.lneg = d_lags_neg;
   ra.ties = p->ties;
   ra.tie_count = p->tie_count;
+  ra.tie_begin = tie_begin;
   ra.tie_cap = p->tie_cap;
   HIP_CHECK(launch_resolve(p->R, ra, 8192, st));
+  return SXC_OK;
+}
+}  // namespace
+
+// FFT form.  d_rev == nullptr: column spectra are the row spectra (lag l of
+// pair (i, j) is sum_n x_i[n] x_j[n - l], circular mod P).  d_rev != nullptr:
+// the reference's formulation X_i R_j with the caller's reversed rows
+// (Correlation_in_OMP/correlation_c.cpp:43-45, :270) -- their spectra,
+// rotated by L - 1, become the column spectra, so lag l reads the reference's
+// corr_t[L - 1 + l] whatever length each event really has.
+int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride,
+                          int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                          hipStream_t st) {
+  hipEvent_t* ev = nullptr;
+  if (p->timing && p->timed < (int)p->events.size() / 4) ev = &p->events[4 * p->timed++];
+  int rc;
+  if (ev) HIP_CHECK(hipEventRecord(ev[0], st));
+  if ((rc = fft_stage_forward(p, d_traces, d_rev, trace_stride, in_len, st))) return rc;
+  if (ev) HIP_CHECK(hipEventRecord(ev[1], st));
+  if ((rc = fft_stage_pairs(p, d_rev != nullptr, p->work, p->n_work, d_vals_pos, d_lags_pos, d_vals_neg,
+                            d_lags_neg, st)))
+    return rc;
+  if (ev) HIP_CHECK(hipEventRecord(ev[2], st));
+  if ((rc = fft_stage_resolve(p, d_traces, d_rev, trace_stride, in_len, nullptr, d_vals_pos, d_lags_pos,
+                              d_vals_neg, d_lags_neg, st)))
+    return rc;
   if (ev) HIP_CHECK(hipEventRecord(ev[3], st));
   return SXC_OK;
 }
 
-static int check_execute_args(sxc_plan* p, const float* d_traces, long long trace_stride, int in_len,
-                              float* vp, int* lp, float* vn, int* ln) {
-  if (!p || !d_traces || !vp || !lp || !vn || !ln) {
-    set_error("null argument");
-    return SXC_ERR_INVALID;
+// Row bands of the packed triangle with about equal pair counts: band b holds
+// rows [band_row[b], band_row[b + 1]), its work items (the plan's chunk-major
+// items of those rows, in the same order) are band_work[band_item[b] ..
+// band_item[b + 1]), and its outputs the packed range [packed(band_row[b]),
+// packed(band_row[b + 1])).  Built on the first banded call.
+int sxc::fft_plan_bands(sxc_plan* p, int n_bands) {
+  if (p->band_work && p->n_bands == n_bands) return SXC_OK;
+  const int n = p->d.n_events;
+  const long long total = (long long)n * (n + 1) / 2;
+  std::vector<int> row(n_bands + 1, n);
+  row[0] = 0;
+  for (int b = 1, r = 0; b < n_bands; ++b) {
+    const long long target = total * b / n_bands;
+    while (r < n && (long long)(r + 1) * n - (long long)(r + 1) * r / 2 <= target) ++r;
+    row[b] = std::max(r, row[b - 1]);
   }
-  if (!p->lag && (in_len < 1 || in_len > p->P || trace_stride < in_len)) {
-    set_error("in_len %d must be in [1, P=%d] and <= trace_stride %lld", in_len, p->P, trace_stride);
-    return SXC_ERR_INVALID;
+  const std::vector<int2> work = build_work(n, p->chunk);
+  std::vector<int2> banded;
+  banded.reserve(work.size());
+  std::vector<int> item(n_bands + 1, 0);
+  for (int b = 0; b < n_bands; ++b) {
+    item[b] = (int)banded.size();
+    for (const int2& w : work)
+      if (w.x >= row[b] && w.x < row[b + 1]) banded.push_back(w);
   }
-  if (p->lag && (in_len > trace_stride || in_len < 1)) {
-    set_error("in_len %d must be in [1, trace_stride %lld]", in_len, trace_stride);
-    return SXC_ERR_INVALID;
-  }
+  item[n_bands] = (int)banded.size();
+  if ((int)banded.size() != p->n_work) { set_error("band work list size mismatch"); return SXC_ERR_INVALID; }
+  if (p->band_work) { (void)hipFree(p->band_work); p->band_work = nullptr; }
+  if (p->band_count) { (void)hipFree(p->band_count); p->band_count = nullptr; }
+  int rc;
+  if ((rc = dev_alloc(&p->band_work, banded.size(), "band work list"))) return rc;
+  if ((rc = dev_alloc(&p->band_count, (size_t)n_bands, "band record counts"))) return rc;
+  HIP_CHECK(hipMemcpy(p->band_work, banded.data(), banded.size() * sizeof(int2), hipMemcpyHostToDevice));
+  p->n_bands = n_bands;
+  p->band_row = row;
+  p->band_item = item;
   return SXC_OK;
+}
+
+// Band b of a banded execute (after fft_stage_forward): the pair kernel over
+// the band's items, then the resolver over the records they appended.
+int sxc::fft_plan_execute_band(sxc_plan* p, int b, const float* d_traces, const float* d_rev,
+                               long long trace_stride, int in_len, float* d_vals_pos, int* d_lags_pos,
+                               float* d_vals_neg, int* d_lags_neg, hipStream_t st) {
+  int rc;
+  if (b == 0 && (rc = fft_stage_forward(p, d_traces, d_rev, trace_stride, in_len, st))) return rc;
+  // the record count before this band's pairs: the band's resolver starts there
+  HIP_CHECK(hipMemcpyAsync(p->band_count + b, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+  const int i0 = p->band_item[b], i1 = p->band_item[b + 1];
+  if ((rc = fft_stage_pairs(p, d_rev != nullptr, p->band_work + i0, i1 - i0, d_vals_pos, d_lags_pos, d_vals_neg,
+                            d_lags_neg, st)))
+    return rc;
+  return fft_stage_resolve(p, d_traces, d_rev, trace_stride, in_len, p->band_count + b, d_vals_pos, d_lags_pos,
+                           d_vals_neg, d_lags_neg, st);
 }
 
 extern "C" {
@@ -815,13 +997,7 @@ int sxc_plan_execute_reversed(sxc_plan* p, const float* d_traces, const float* d
   clear_error();
   int rc = check_execute_args(p, d_traces, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg, d_lags_neg);
   if (rc) return rc;
-  if (!d_reversed) { set_error("null argument"); return SXC_ERR_INVALID; }
-  if (p->lag) { set_error("reversed rows are an FFT-form execute (the lag forms correlate the traces directly)"); return SXC_ERR_INVALID; }
-  if (p->d.demean) { set_error("reversed rows cannot be demeaned consistently: create the plan with demean = 0"); return SXC_ERR_INVALID; }
-  if (!p->col_spec &&
-      (rc = dev_alloc(&p->col_spec, (size_t)((long long)p->d.n_events * p->d.n_channels * p->spec_stride),
-                      "reversed-row spectra")))
-    return rc;
+  if ((rc = prepare_execute_reversed(p, d_traces, d_reversed, trace_stride, in_len))) return rc;
   hipStream_t st = (hipStream_t)stream;
   p->last_stream = st;
   return fft_plan_execute(p, d_traces, d_reversed, trace_stride, in_len, d_vals_pos, d_lags_pos, d_vals_neg,

@@ -166,22 +166,38 @@ void release(std::unique_ptr<Stager> s) {
 
 }  // namespace
 
-int copy_to_device(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  if (bytes == 0) return SXC_OK;
+namespace {
+struct Chunk {
+  char* host;
+  char* dev;
+  size_t n;
+};
+// The segments cut into staging-sized chunks, in order: one pipeline over
+// all of them, so consecutive arrays leave no gap between their DMAs.
+std::vector<Chunk> chunks_of(const HostSeg* segs, int n_segs) {
+  std::vector<Chunk> c;
+  for (int s = 0; s < n_segs; ++s)
+    for (size_t off = 0; off < segs[s].bytes; off += kChunk)
+      c.push_back({(char*)segs[s].host + off, (char*)segs[s].dev + off, std::min(kChunk, segs[s].bytes - off)});
+  return c;
+}
+}  // namespace
+
+int copy_segs_to_device(const HostSeg* segs, int n_segs, hipStream_t st) {
+  const std::vector<Chunk> c = chunks_of(segs, n_segs);
+  if (c.empty()) return SXC_OK;
   std::unique_ptr<Stager> s = acquire();
   if (!s) return SXC_ERR_HIP;
-  const size_t n_chunks = (bytes + kChunk - 1) / kChunk;
   hipError_t e = hipSuccess;
-  for (size_t k = 0; k < n_chunks && e == hipSuccess; ++k) {
+  for (size_t k = 0; k < c.size() && e == hipSuccess; ++k) {
     const int b = (int)(k & 1);
-    const size_t off = k * kChunk, n = std::min(kChunk, bytes - off);
     // the DMA of chunk k - 2 out of this buffer must be done
     if (k >= 2 && (e = hipEventSynchronize(s->ev[b])) != hipSuccess) break;
-    CopyPool::get().copy(s->buf[b], (const char*)src + off, n);
-    if ((e = hipMemcpyAsync((char*)dst + off, s->buf[b], n, hipMemcpyHostToDevice, st)) != hipSuccess) break;
+    CopyPool::get().copy(s->buf[b], c[k].host, c[k].n);
+    if ((e = hipMemcpyAsync(c[k].dev, s->buf[b], c[k].n, hipMemcpyHostToDevice, st)) != hipSuccess) break;
     e = hipEventRecord(s->ev[b], st);
   }
-  if (e == hipSuccess) e = hipEventSynchronize(s->ev[(n_chunks - 1) & 1]);
+  if (e == hipSuccess) e = hipEventSynchronize(s->ev[(c.size() - 1) & 1]);
   if (e != hipSuccess) {
     set_error("copying to the device failed: %s", hipGetErrorString(e));
     (void)hipStreamSynchronize(st);  // no DMA may still read the staging buffers
@@ -192,25 +208,22 @@ int copy_to_device(void* dst, const void* src, size_t bytes, hipStream_t st) {
   return SXC_OK;
 }
 
-int copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
-  if (bytes == 0) return SXC_OK;
+int copy_segs_to_host(const HostSeg* segs, int n_segs, hipStream_t st) {
+  const std::vector<Chunk> c = chunks_of(segs, n_segs);
+  if (c.empty()) return SXC_OK;
   std::unique_ptr<Stager> s = acquire();
   if (!s) return SXC_ERR_HIP;
-  const size_t n_chunks = (bytes + kChunk - 1) / kChunk;
   auto issue = [&](size_t k) -> hipError_t {
     const int b = (int)(k & 1);
-    const size_t off = k * kChunk, n = std::min(kChunk, bytes - off);
-    hipError_t e = hipMemcpyAsync(s->buf[b], (const char*)src + off, n, hipMemcpyDeviceToHost, st);
+    hipError_t e = hipMemcpyAsync(s->buf[b], c[k].dev, c[k].n, hipMemcpyDeviceToHost, st);
     return e == hipSuccess ? hipEventRecord(s->ev[b], st) : e;
   };
   hipError_t e = issue(0);
-  for (size_t k = 0; k < n_chunks && e == hipSuccess; ++k) {
+  for (size_t k = 0; k < c.size() && e == hipSuccess; ++k) {
     // chunk k + 1 goes into the other buffer, whose chunk k - 1 was copied out
-    if (k + 1 < n_chunks && (e = issue(k + 1)) != hipSuccess) break;
-    const int b = (int)(k & 1);
-    if ((e = hipEventSynchronize(s->ev[b])) != hipSuccess) break;
-    const size_t off = k * kChunk, n = std::min(kChunk, bytes - off);
-    CopyPool::get().copy((char*)dst + off, s->buf[b], n);
+    if (k + 1 < c.size() && (e = issue(k + 1)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(s->ev[k & 1])) != hipSuccess) break;
+    CopyPool::get().copy(c[k].host, s->buf[k & 1], c[k].n);
   }
   if (e != hipSuccess) {
     set_error("copying to the host failed: %s", hipGetErrorString(e));
@@ -220,6 +233,16 @@ int copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
   }
   release(std::move(s));
   return SXC_OK;
+}
+
+int copy_to_device(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  const HostSeg seg{(void*)src, dst, bytes};
+  return copy_segs_to_device(&seg, 1, st);
+}
+
+int copy_to_host(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  const HostSeg seg{dst, (void*)src, bytes};
+  return copy_segs_to_host(&seg, 1, st);
 }
 
 }  // namespace sxc

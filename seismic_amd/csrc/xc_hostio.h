@@ -7,6 +7,17 @@
 #include <cstddef>
 
 namespace sxc {
+// One array of a multi-array copy: `bytes` at `host` (pageable) and `dev`.
+struct HostSeg {
+  void* host;
+  void* dev;
+  size_t bytes;
+};
+// Copies several arrays (host -> device, device -> host) as one pipeline of
+// staged chunks, in the order given, on `st`; returns when all are in place.
+// SXC_OK, or an error code with the error string set.
+int copy_segs_to_device(const HostSeg* segs, int n_segs, hipStream_t st);
+int copy_segs_to_host(const HostSeg* segs, int n_segs, hipStream_t st);
 // Copies `bytes` from pageable host memory to device memory on `st` and
 // returns when the copy is complete.  The data goes through two pinned
 // staging buffers: the host-side copy of one chunk (several threads) runs

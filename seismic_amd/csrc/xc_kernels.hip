@@ -654,7 +654,11 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   constexpr int P = 2 * M;
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
-  const unsigned total = min(*a.tie_count, (unsigned)a.tie_cap);
+  // records [begin, end): tie_begin, when set, holds the count before the
+  // pair launches whose records this launch resolves (row bands)
+  const unsigned end = min(*a.tie_count, (unsigned)a.tie_cap);
+  const unsigned begin = a.tie_begin ? min(*a.tie_begin, end) : 0u;
+  const unsigned total = end - begin;
   // XCD-aware walk: XCD x = blockIdx % 8 resolves its own contiguous eighth
   // of the record list (records arrive roughly grouped by tile or row), so
   // the traces a record re-reads are usually still in that XCD's L2
@@ -668,7 +672,7 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   float2* wl = smem + TwLayoutCT<R>::kInv;
   stage_twiddles<R>(twl, a.tw + TwLayout<R>::kInv, TwLayoutCT<R>::kInv, lane, 64);
   for (unsigned rec = rec0; rec < seg_end; rec += n_local) {
-    const int4 t = a.ties[rec];
+    const int4 t = a.ties[begin + rec];
     const int i = t.x, j = t.y, side = t.z;
     wave_lds_fence();
     cf y[R];

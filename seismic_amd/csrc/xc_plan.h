@@ -63,6 +63,12 @@ struct sxc_plan {
   double* fnorms = nullptr;
   int4* wide = nullptr;
   unsigned* wide_count = nullptr;
+  // row bands (host-buffer calls: outputs copied out band by band while the
+  // next band computes), built on first use by fft_plan_bands
+  int n_bands = 0;
+  int2* band_work = nullptr;
+  unsigned* band_count = nullptr;  // [n_bands] record count before each band
+  std::vector<int> band_row, band_item;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -77,6 +83,14 @@ void fft_plan_geometry(const sxc_plan_desc& desc, long long tie_cap_max, sxc_pla
 // FP64 resolver on the plan's workspace (asynchronous on st).
 int fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride, int in_len,
                      float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg, hipStream_t st);
+// FFT form in row bands: fft_plan_bands prepares n_bands bands (packed-pair
+// counts about equal); fft_plan_execute_band(b) for b = 0, 1, ... in order on
+// one stream runs the forward stage (b = 0), then band b's pairs and their
+// near ties.  Same outputs as fft_plan_execute.
+int fft_plan_bands(sxc_plan* p, int n_bands);
+int fft_plan_execute_band(sxc_plan* p, int b, const float* d_traces, const float* d_rev, long long trace_stride,
+                          int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                          hipStream_t st);
 // lane masks of the in-window slots of the inverse wave FFT of R values per lane
 std::vector<unsigned long long> window_masks(int R, int lag_lo, int lag_hi);
 }  // namespace sxc
