@@ -27,7 +27,7 @@ namespace sxc {
 namespace {
 
 constexpr size_t kChunk = 8u << 20;  // bytes per staged chunk
-constexpr int kCopyThreads = 4;      // host threads per chunk copy (the caller is one of them)
+constexpr int kCopyThreads = 4;      // at most this many host threads per chunk copy (the caller is one)
 constexpr size_t kMinSplit = 1u << 20;
 
 // Persistent workers for the host-side chunk copies.  One job at a time
@@ -40,7 +40,8 @@ class CopyPool {
   }
 
   void copy(void* dst, const void* src, size_t n) {
-    const int parts = n < kMinSplit ? 1 : kCopyThreads;
+    // one part per thread: the caller's and one per worker
+    const int parts = n < kMinSplit ? 1 : 1 + (int)workers_.size();
     if (parts == 1 || workers_.empty()) {
       memcpy(dst, src, n);
       return;
