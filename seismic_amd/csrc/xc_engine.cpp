@@ -593,7 +593,7 @@ static int prepare_execute_reversed(sxc_plan* p, const float* d_traces, const fl
 // the pair kernel and the resolver of band b + 1 run while band b's packed
 // outputs are copied to the host (the copies of the last band alone are not
 // hidden).
-static constexpr int kBands = 4;
+static constexpr int kBands = 8;
 static constexpr int kBandMinEvents = 1024;
 
 // Banded execute and output copies of a host-buffer job (inputs on the device).
