@@ -121,7 +121,7 @@ void plan_free(sxc_plan* p) {
   for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
   void* bufs[] = {p->spec, p->col_spec, p->norms, p->traces, p->work, p->ties, p->tie_count,
                   p->fwd2, p->inv2, p->post, p->winmask, p->hi, p->lo, p->lag_traces, p->tiles,
-                  p->fnorms, p->wide, p->wide_count, p->band_work, p->band_count};
+                  p->fnorms, p->wide, p->wide_count, p->blk_work, p->grp_count};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -461,7 +461,7 @@ static size_t plan_device_bytes(const sxc_plan* p) {
   size_t b = rows * (size_t)p->spec_stride * sizeof(float2) * ((p->spec ? 1 : 0) + (p->col_spec ? 1 : 0));
   b += rows * sizeof(double) * ((p->norms ? 1 : 0) + (p->fnorms ? 1 : 0));
   if (p->traces) b += rows * (size_t)p->P * sizeof(float);
-  b += (size_t)p->n_work * sizeof(int2) * (p->band_work ? 2 : 1) +
+  b += (size_t)p->n_work * sizeof(int2) * (p->blk_work ? 2 : 1) +
        (size_t)p->tie_cap * sizeof(int4) * (p->lag ? 3 : 1);
   if (p->lag) b += rows * (size_t)p->ls * 2 * (p->lo ? 2 : 1) + rows * (size_t)p->d.trace_len * sizeof(float);
   return b;
@@ -589,45 +589,66 @@ static int prepare_execute_reversed(sxc_plan* p, const float* d_traces, const fl
   return SXC_OK;
 }
 
-// Host-buffer FFT-form runs of at least this many events go in row bands:
-// the pair kernel and the resolver of band b + 1 run while band b's packed
-// outputs are copied to the host (the copies of the last band alone are not
-// hidden).
+// Host-buffer FFT-form runs of at least this many events are blocked in
+// event bands (fft_plan_blocks): band b's traces are copied in while the
+// pairs of column band b - 1 compute, and once the last band is in, row band
+// a's outputs are copied out while row band a + 1 completes.  Only the first
+// band's input copy and the last row band's output copy are not hidden.
 static constexpr int kBands = 8;
 static constexpr int kBandMinEvents = 1024;
 
-// Banded execute and output copies of a host-buffer job (inputs on the device).
-static int run_banded(HostJob* j, bool reversed, long long trace_stride, int in_len, float* vals_pos,
-                      int* lags_pos, float* vals_neg, int* lags_neg) {
+// Blocked run of a host-buffer job: input copies, compute and output copies.
+static int run_blocked(HostJob* j, const float* traces, const float* reversed, long long trace_stride, int in_len,
+                       float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg) {
   sxc_plan* p = j->p;
   int rc;
-  if ((rc = fft_plan_bands(p, kBands))) return rc;
+  if ((rc = fft_plan_blocks(p, kBands))) return rc;
   if (!j->cs) HIP_CHECK(hipStreamCreateWithFlags(&j->cs, hipStreamNonBlocking));
   if (!j->xs) HIP_CHECK(hipStreamCreateWithFlags(&j->xs, hipStreamNonBlocking));
-  while ((int)j->band_ev.size() < kBands) {
+  while ((int)j->band_ev.size() < 2 * kBands) {
     hipEvent_t e;
     HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     j->band_ev.push_back(e);
   }
   p->last_stream = j->cs;
+  const float* d_rev = reversed ? j->rev : nullptr;
+  const int C = p->d.n_channels;
+  HIP_CHECK(launch_fill_u32(p->tie_count, 0u, 1, j->cs));
   for (int b = 0; b < kBands; ++b) {
-    if ((rc = fft_plan_execute_band(p, b, j->in, reversed ? j->rev : nullptr, trace_stride, in_len, j->vp, j->lp,
-                                    j->vn, j->ln, j->cs)))
-      return rc;
-    HIP_CHECK(hipEventRecord(j->band_ev[b], j->cs));
+    // event band b in (copy stream), then its transforms and pairs (compute stream)
+    const int e0 = p->band_ev[b], e1 = p->band_ev[b + 1];
+    const size_t off = (size_t)e0 * C * trace_stride, cnt = (size_t)(e1 - e0) * C * trace_stride;
+    const HostSeg in[2] = {{(void*)(traces + off), j->in + off, cnt * sizeof(float)},
+                           {(void*)(reversed ? reversed + off : nullptr), reversed ? j->rev + off : nullptr,
+                            cnt * sizeof(float)}};
+    if ((rc = copy_segs_to_device(in, reversed ? 2 : 1, j->xs))) break;
+    HIP_CHECK(hipEventRecord(j->band_ev[b], j->xs));
+    HIP_CHECK(hipStreamWaitEvent(j->cs, j->band_ev[b], 0));
+    if ((rc = fft_plan_forward_events(p, e0, e1, j->in, d_rev, trace_stride, in_len, j->cs))) break;
+    if (b < kBands - 1) {
+      rc = fft_plan_group(p, b, j->in, d_rev, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln, j->cs);
+    } else {
+      for (int a = 0; a < kBands && !rc; ++a) {
+        rc = fft_plan_group(p, kBands - 1 + a, j->in, d_rev, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln,
+                            j->cs);
+        if (!rc) HIP_CHECK(hipEventRecord(j->band_ev[kBands + a], j->cs));
+      }
+    }
+    if (rc) break;
   }
   const long long n = p->d.n_events;
   auto packed = [n](long long r) { return r * n - r * (r - 1) / 2; };  // first index of row r
-  for (int b = 0; b < kBands && !rc; ++b) {
-    const size_t o = (size_t)packed(p->band_row[b]), c = (size_t)packed(p->band_row[b + 1]) - o;
-    HIP_CHECK(hipStreamWaitEvent(j->xs, j->band_ev[b], 0));
-    const HostSeg segs[4] = {{vals_pos + o, j->vp + o, c * sizeof(float)},
-                             {lags_pos + o, j->lp + o, c * sizeof(int)},
-                             {vals_neg + o, j->vn + o, c * sizeof(float)},
-                             {lags_neg + o, j->ln + o, c * sizeof(int)}};
-    rc = copy_segs_to_host(segs, 4, j->xs);
+  for (int a = 0; a < kBands && !rc; ++a) {
+    const size_t o = (size_t)packed(p->band_ev[a]), c = (size_t)packed(p->band_ev[a + 1]) - o;
+    HIP_CHECK(hipStreamWaitEvent(j->xs, j->band_ev[kBands + a], 0));
+    const HostSeg out[4] = {{vals_pos + o, j->vp + o, c * sizeof(float)},
+                            {lags_pos + o, j->lp + o, c * sizeof(int)},
+                            {vals_neg + o, j->vn + o, c * sizeof(float)},
+                            {lags_neg + o, j->ln + o, c * sizeof(int)}};
+    rc = copy_segs_to_host(out, 4, j->xs);
   }
   hipError_t es = hipStreamSynchronize(j->cs);
+  if (es == hipSuccess) es = hipStreamSynchronize(j->xs);
   if (es != hipSuccess) {
     set_error("device execution failed: %s", hipGetErrorString(es));
     return SXC_ERR_HIP;
@@ -655,17 +676,11 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
   const bool banded = !j->p->lag && desc->n_events >= kBandMinEvents;
   const HostSeg in_segs[2] = {{(void*)traces, j->in, in_bytes}, {(void*)reversed, j->rev, in_bytes}};
   if (banded) {
-    // the checks and lazy set-up of sxc_plan_execute(_reversed), then the bands
+    // the checks and lazy set-up of sxc_plan_execute(_reversed), then the blocked run
     rc = reversed ? prepare_execute_reversed(j->p, j->in, j->rev, trace_stride, in_len)
                   : check_execute_args(j->p, j->in, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln);
-    if (!rc && !j->cs) {
-      if (hipStreamCreateWithFlags(&j->cs, hipStreamNonBlocking) != hipSuccess) {
-        set_error("creating a stream failed");
-        rc = SXC_ERR_HIP;
-      }
-    }
-    if (!rc) rc = copy_segs_to_device(in_segs, reversed ? 2 : 1, j->cs);
-    if (!rc) rc = run_banded(j, reversed != nullptr, trace_stride, in_len, vals_pos, lags_pos, vals_neg, lags_neg);
+    if (!rc)
+      rc = run_blocked(j, traces, reversed, trace_stride, in_len, vals_pos, lags_pos, vals_neg, lags_neg);
     const int tie_rc = rc == SXC_OK ? check_tie_overflow(j->p) : SXC_OK;
     if (rc == SXC_ERR_HIP) job_free(j);  // a failed device may have left the job's state unusable
     else job_release(j);
@@ -918,59 +933,101 @@ int sxc::fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev
   return SXC_OK;
 }
 
-// Row bands of the packed triangle with about equal pair counts: band b holds
-// rows [band_row[b], band_row[b + 1]), its work items (the plan's chunk-major
-// items of those rows, in the same order) are band_work[band_item[b] ..
-// band_item[b + 1]), and its outputs the packed range [packed(band_row[b]),
-// packed(band_row[b + 1])).  Built on the first banded call.
-int sxc::fft_plan_bands(sxc_plan* p, int n_bands) {
-  if (p->band_work && p->n_bands == n_bands) return SXC_OK;
-  const int n = p->d.n_events;
-  const long long total = (long long)n * (n + 1) / 2;
-  std::vector<int> row(n_bands + 1, n);
-  row[0] = 0;
-  for (int b = 1, r = 0; b < n_bands; ++b) {
-    const long long target = total * b / n_bands;
-    while (r < n && (long long)(r + 1) * n - (long long)(r + 1) * r / 2 <= target) ++r;
-    row[b] = std::max(r, row[b - 1]);
+// Blocked execute for host-buffer calls.  Event band k = events
+// [band_ev[k], band_ev[k + 1]) (multiples of the column chunk but the last),
+// B bands.  Launch groups, in execution order, over the plan's work items
+// (chunk-major within a group):
+//   g = b < B - 1: every pair of column band b (rows of bands 0 .. b), run as
+//                  soon as band b's traces are on the device;
+//   g = B - 1 + a: the pairs of row band a in column band B - 1; after it, row
+//                  band a's packed outputs (contiguous) are final.
+// blk_work[grp_item[g] .. grp_item[g + 1]) are group g's items.
+int sxc::fft_plan_blocks(sxc_plan* p, int B) {
+  if (p->blk_work && p->n_bands == B) return SXC_OK;
+  const int n = p->d.n_events, ch = p->chunk;
+  std::vector<int> ev(B + 1, n);
+  ev[0] = 0;
+  for (int k = 1; k < B; ++k) ev[k] = std::min(n, std::max(ev[k - 1], (int)((long long)n * k / B / ch * ch)));
+  auto band_of = [&](int e) {
+    int k = 0;
+    while (k + 1 < B && e >= ev[k + 1]) ++k;
+    return k;
+  };
+  const int n_groups = 2 * B - 1;
+  std::vector<std::vector<int2>> grp(n_groups);
+  for (const int2& w : build_work(n, ch)) {
+    const int cb = band_of(w.y / ch * ch), rb = band_of(w.x);
+    grp[cb < B - 1 ? cb : B - 1 + rb].push_back(w);
   }
-  const std::vector<int2> work = build_work(n, p->chunk);
-  std::vector<int2> banded;
-  banded.reserve(work.size());
-  std::vector<int> item(n_bands + 1, 0);
-  for (int b = 0; b < n_bands; ++b) {
-    item[b] = (int)banded.size();
-    for (const int2& w : work)
-      if (w.x >= row[b] && w.x < row[b + 1]) banded.push_back(w);
+  std::vector<int2> all;
+  std::vector<int> item(n_groups + 1, 0);
+  for (int g = 0; g < n_groups; ++g) {
+    item[g] = (int)all.size();
+    all.insert(all.end(), grp[g].begin(), grp[g].end());
   }
-  item[n_bands] = (int)banded.size();
-  if ((int)banded.size() != p->n_work) { set_error("band work list size mismatch"); return SXC_ERR_INVALID; }
-  if (p->band_work) { (void)hipFree(p->band_work); p->band_work = nullptr; }
-  if (p->band_count) { (void)hipFree(p->band_count); p->band_count = nullptr; }
+  item[n_groups] = (int)all.size();
+  if ((int)all.size() != p->n_work) { set_error("blocked work list size mismatch"); return SXC_ERR_INVALID; }
+  if (p->blk_work) { (void)hipFree(p->blk_work); p->blk_work = nullptr; }
+  if (p->grp_count) { (void)hipFree(p->grp_count); p->grp_count = nullptr; }
   int rc;
-  if ((rc = dev_alloc(&p->band_work, banded.size(), "band work list"))) return rc;
-  if ((rc = dev_alloc(&p->band_count, (size_t)n_bands, "band record counts"))) return rc;
-  HIP_CHECK(hipMemcpy(p->band_work, banded.data(), banded.size() * sizeof(int2), hipMemcpyHostToDevice));
-  p->n_bands = n_bands;
-  p->band_row = row;
-  p->band_item = item;
+  if ((rc = dev_alloc(&p->blk_work, all.size(), "blocked work list"))) return rc;
+  if ((rc = dev_alloc(&p->grp_count, (size_t)n_groups, "group record counts"))) return rc;
+  HIP_CHECK(hipMemcpy(p->blk_work, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+  p->n_bands = B;
+  p->band_ev = ev;
+  p->grp_item = item;
   return SXC_OK;
 }
 
-// Band b of a banded execute (after fft_stage_forward): the pair kernel over
-// the band's items, then the resolver over the records they appended.
-int sxc::fft_plan_execute_band(sxc_plan* p, int b, const float* d_traces, const float* d_rev,
-                               long long trace_stride, int in_len, float* d_vals_pos, int* d_lags_pos,
-                               float* d_vals_neg, int* d_lags_neg, hipStream_t st) {
+// Forward (and reversed-row) R2C of the rows of events [e0, e1).
+int sxc::fft_plan_forward_events(sxc_plan* p, int e0, int e1, const float* d_traces, const float* d_rev,
+                                 long long trace_stride, int in_len, hipStream_t st) {
+  const sxc_plan_desc& d = p->d;
+  const long long r0 = (long long)e0 * d.n_channels;
+  ForwardArgs fa{};
+  fa.in = d_traces + r0 * trace_stride;
+  fa.in_stride = trace_stride;
+  fa.in_len = in_len;
+  fa.L = d.trace_len;
+  fa.n_rows = (e1 - e0) * d.n_channels;
+  fa.n_channels = d.n_channels;
+  fa.demean = d.demean;
+  fa.norms = p->norms + r0;
+  fa.out_traces = p->traces ? p->traces + r0 * p->P : nullptr;
+  fa.out_stride = p->P;
+  fa.spec = p->spec + r0 * p->spec_stride;
+  fa.spec_stride = p->spec_stride;
+  fa.tw = p->fwd2;
+  fa.post_tw = p->post;
+  if (fa.n_rows <= 0) return SXC_OK;
+  HIP_CHECK(launch_forward(p->R, fa, st));
+  if (d_rev) {
+    ForwardArgs ra = fa;
+    ra.in = d_rev + r0 * trace_stride;
+    ra.demean = 0;
+    ra.norms = nullptr;
+    ra.out_traces = nullptr;
+    ra.spec = p->col_spec + r0 * p->spec_stride;
+    ra.reversed = 1;
+    ra.rev_shift = d.trace_len - 1;
+    ra.norms_in = p->norms + r0;
+    HIP_CHECK(launch_forward(p->R, ra, st));
+  }
+  return SXC_OK;
+}
+
+// Launch group g of a blocked execute: its pairs, then the resolver over the
+// near-tie records they appended (from the count before the group).
+int sxc::fft_plan_group(sxc_plan* p, int g, const float* d_traces, const float* d_rev, long long trace_stride,
+                        int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                        hipStream_t st) {
+  HIP_CHECK(hipMemcpyAsync(p->grp_count + g, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+  const int i0 = p->grp_item[g], i1 = p->grp_item[g + 1];
   int rc;
-  if (b == 0 && (rc = fft_stage_forward(p, d_traces, d_rev, trace_stride, in_len, st))) return rc;
-  // the record count before this band's pairs: the band's resolver starts there
-  HIP_CHECK(hipMemcpyAsync(p->band_count + b, p->tie_count, sizeof(unsigned), hipMemcpyDeviceToDevice, st));
-  const int i0 = p->band_item[b], i1 = p->band_item[b + 1];
-  if ((rc = fft_stage_pairs(p, d_rev != nullptr, p->band_work + i0, i1 - i0, d_vals_pos, d_lags_pos, d_vals_neg,
+  if ((rc = fft_stage_pairs(p, d_rev != nullptr, p->blk_work + i0, i1 - i0, d_vals_pos, d_lags_pos, d_vals_neg,
                             d_lags_neg, st)))
     return rc;
-  return fft_stage_resolve(p, d_traces, d_rev, trace_stride, in_len, p->band_count + b, d_vals_pos, d_lags_pos,
+  return fft_stage_resolve(p, d_traces, d_rev, trace_stride, in_len, p->grp_count + g, d_vals_pos, d_lags_pos,
                            d_vals_neg, d_lags_neg, st);
 }
 

@@ -85,7 +85,7 @@ struct ResolveArgs {
   int* lneg;
   const int4* ties;
   const unsigned* tie_count;
-  const unsigned* tie_begin;  // null: records from 0; else from *tie_begin (row-band launches)
+  const unsigned* tie_begin;  // null: records from 0; else from *tie_begin (launch groups of a blocked call)
   int tie_cap;
   long long out_ld;       // 0: packed triangle output; else dense block rows of this stride
   int out_row0, out_col0; // event index of the block's first row / column

@@ -655,7 +655,8 @@ __global__ __launch_bounds__(64) void k_resolve(ResolveArgs a) {
   extern __shared__ float2 smem[];
   const int lane = threadIdx.x;
   // records [begin, end): tie_begin, when set, holds the count before the
-  // pair launches whose records this launch resolves (row bands)
+  // pair launches whose records this launch resolves (the launch groups of
+  // a blocked host-buffer call)
   const unsigned end = min(*a.tie_count, (unsigned)a.tie_cap);
   const unsigned begin = a.tie_begin ? min(*a.tie_begin, end) : 0u;
   const unsigned total = end - begin;

@@ -63,12 +63,12 @@ struct sxc_plan {
   double* fnorms = nullptr;
   int4* wide = nullptr;
   unsigned* wide_count = nullptr;
-  // row bands (host-buffer calls: outputs copied out band by band while the
-  // next band computes), built on first use by fft_plan_bands
+  // blocked execute of host-buffer calls (fft_plan_blocks): event bands,
+  // launch groups and the record count before each group
   int n_bands = 0;
-  int2* band_work = nullptr;
-  unsigned* band_count = nullptr;  // [n_bands] record count before each band
-  std::vector<int> band_row, band_item;
+  int2* blk_work = nullptr;
+  unsigned* grp_count = nullptr;
+  std::vector<int> band_ev, grp_item;
   // stage timing: 4 events per execute (start, after forward, pairs, resolve)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -83,14 +83,19 @@ void fft_plan_geometry(const sxc_plan_desc& desc, long long tie_cap_max, sxc_pla
 // FP64 resolver on the plan's workspace (asynchronous on st).
 int fft_plan_execute(sxc_plan* p, const float* d_traces, const float* d_rev, long long trace_stride, int in_len,
                      float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg, hipStream_t st);
-// FFT form in row bands: fft_plan_bands prepares n_bands bands (packed-pair
-// counts about equal); fft_plan_execute_band(b) for b = 0, 1, ... in order on
-// one stream runs the forward stage (b = 0), then band b's pairs and their
-// near ties.  Same outputs as fft_plan_execute.
-int fft_plan_bands(sxc_plan* p, int n_bands);
-int fft_plan_execute_band(sxc_plan* p, int b, const float* d_traces, const float* d_rev, long long trace_stride,
-                          int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
-                          hipStream_t st);
+// FFT form, blocked (host-buffer calls): fft_plan_blocks prepares B event
+// bands and the 2B - 1 launch groups; with event band b's traces on the
+// device, fft_plan_forward_events transforms them and fft_plan_group(b) (b <
+// B - 1) computes column band b; after the last band, fft_plan_group(B - 1 +
+// a) completes row band a.  Each group resolves its own near ties; the
+// outputs equal fft_plan_execute's.  The near-tie counter must be zeroed
+// before the first group.
+int fft_plan_blocks(sxc_plan* p, int B);
+int fft_plan_forward_events(sxc_plan* p, int e0, int e1, const float* d_traces, const float* d_rev,
+                            long long trace_stride, int in_len, hipStream_t st);
+int fft_plan_group(sxc_plan* p, int g, const float* d_traces, const float* d_rev, long long trace_stride,
+                   int in_len, float* d_vals_pos, int* d_lags_pos, float* d_vals_neg, int* d_lags_neg,
+                   hipStream_t st);
 // lane masks of the in-window slots of the inverse wave FFT of R values per lane
 std::vector<unsigned long long> window_masks(int R, int lag_lo, int lag_hi);
 }  // namespace sxc

@@ -1,9 +1,11 @@
-"""Host-buffer calls of at least 1024 events run the pair kernel and the
-resolver in row bands and copy each band's packed outputs to the host while
-the next band computes.  Same kernels per pair and the same near-tie
-records, so the outputs must equal the single-launch device execute's bit
-for bit: sxc_allpairs_host against sxc_plan_execute, and the drop-in
-correlationCUDA (reversed rows) against sxc_plan_execute_reversed."""
+"""Host-buffer calls of at least 1024 events run blocked in event bands:
+inputs copied in band by band while earlier bands compute, the last column
+band completed row band by row band while finished row bands are copied
+out, each launch group resolving its own near ties.  Same kernels per pair
+and the same near-tie records, so the outputs must equal the single-launch
+device execute's bit for bit: sxc_allpairs_host against sxc_plan_execute,
+and the drop-in correlationCUDA (reversed rows) against
+sxc_plan_execute_reversed."""
 import numpy as np
 import pytest
 
@@ -31,7 +33,7 @@ def test_banded_host_call_equals_device_execute(n, C, L, P, lo, hi):
     print(f"case banded host call N={n} C={C} L={L} P={P} lags [{lo}, {hi}]", flush=True)
     got = engine.allpairs_host(ev, P, lo, hi)
     want, st = _device_execute(ev, P, lo, hi, L)
-    assert st["near_ties"] > 0  # the bands' resolver launches have records to settle
+    assert st["near_ties"] > 0  # the groups' resolver launches have records to settle
     for name, g, w in zip(("vals_pos", "lags_pos", "vals_neg", "lags_neg"), got, want):
         assert np.array_equal(g, w), name
 
