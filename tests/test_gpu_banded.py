@@ -15,9 +15,9 @@ from seismic_amd.device import DeviceArray, synchronize
 pytestmark = pytest.mark.gpu
 
 
-def _device_execute(rows, P, lo, hi, trace_len, rev=None):
+def _device_execute(rows, P, lo, hi, trace_len, rev=None, threshold=0.0, demean=False):
     n, C, _ = rows.shape
-    plan = engine.AllPairsPlan(n, C, trace_len, P, lo, hi)
+    plan = engine.AllPairsPlan(n, C, trace_len, P, lo, hi, threshold=threshold, demean=demean)
     tr = DeviceArray.from_numpy(rows)
     out = plan.execute(tr) if rev is None else plan.execute_reversed(tr, DeviceArray.from_numpy(rev))
     synchronize()
@@ -47,5 +47,19 @@ def test_banded_dropin_equals_device_execute_reversed():
     _, L_, P, padded, rev, _ = correlation_lib.prepare(x)
     assert P == 2048
     want, _ = _device_execute(padded[:, None, :], P, -L_ + 1, L_, L_, rev=rev[:, None, :])
+    for name, g, w in zip(("vals_pos", "lags_pos", "vals_neg", "lags_neg"), got, want):
+        assert np.array_equal(g, w), name
+
+
+def test_banded_host_call_demean_threshold():
+    """FP64 demeaning in the blocked run (each band's transform writes its
+    rows of the demeaned copy the resolver reads) and a threshold."""
+    n, C, L, P, lo, hi, thr = 1100, 2, 900, 2048, -600, 400, 0.15
+    ev, _ = synth.make_events(n, C, L, seed=1100)
+    ev = ev + np.float32(0.3)  # a mean for the demeaning to remove
+    print(f"case banded host call with demean N={n} C={C} L={L} P={P} thr={thr}", flush=True)
+    got = engine.allpairs_host(ev, P, lo, hi, threshold=thr, demean=True)
+    want, st = _device_execute(ev, P, lo, hi, L, threshold=thr, demean=True)
+    assert st["near_ties"] > 0
     for name, g, w in zip(("vals_pos", "lags_pos", "vals_neg", "lags_neg"), got, want):
         assert np.array_equal(g, w), name
