@@ -597,9 +597,18 @@ static int prepare_execute_reversed(sxc_plan* p, const float* d_traces, const fl
 static constexpr int kBands = 8;
 static constexpr int kBandMinEvents = 1024;
 
+// SEISMIC_XCORR_BLOCKED=0 runs every host-buffer call unblocked (read per call).
+static bool blocking_enabled() {
+  const char* e = getenv("SEISMIC_XCORR_BLOCKED");
+  return !(e && e[0] == '0');
+}
+
 // Blocked run of a host-buffer job: input copies, compute and output copies.
+// hist (optional, the OMP ABI): its counting and threshold pass runs over
+// each row band's packed range once the band is final, before the copy.
 static int run_blocked(HostJob* j, const float* traces, const float* reversed, long long trace_stride, int in_len,
-                       float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg) {
+                       float* vals_pos, int* lags_pos, float* vals_neg, int* lags_neg,
+                       const HistArgs* hist = nullptr) {
   sxc_plan* p = j->p;
   int rc;
   if ((rc = fft_plan_blocks(p, kBands))) return rc;
@@ -631,6 +640,17 @@ static int run_blocked(HostJob* j, const float* traces, const float* reversed, l
       for (int a = 0; a < kBands && !rc; ++a) {
         rc = fft_plan_group(p, kBands - 1 + a, j->in, d_rev, trace_stride, in_len, j->vp, j->lp, j->vn, j->ln,
                             j->cs);
+        if (!rc && hist) {
+          const long long n = p->d.n_events, r0 = p->band_ev[a], r1 = p->band_ev[a + 1];
+          const long long o = r0 * n - r0 * (r0 - 1) / 2, c = r1 * n - r1 * (r1 - 1) / 2 - o;
+          HistArgs ha = *hist;
+          ha.vpos += o;
+          ha.lpos += o;
+          ha.vneg += o;
+          ha.lneg += o;
+          ha.n = c;
+          HIP_CHECK(launch_hist_threshold(ha, j->cs));
+        }
         if (!rc) HIP_CHECK(hipEventRecord(j->band_ev[kBands + a], j->cs));
       }
     }
@@ -673,7 +693,7 @@ static int allpairs_host(const sxc_plan_desc* desc, const float* traces, const f
   const long long rows = (long long)desc->n_events * desc->n_channels;
   const size_t in_bytes = (size_t)(rows * trace_stride) * sizeof(float);
   const size_t np = (size_t)j->p->n_pairs;
-  const bool banded = !j->p->lag && desc->n_events >= kBandMinEvents;
+  const bool banded = !j->p->lag && desc->n_events >= kBandMinEvents && blocking_enabled();
   const HostSeg in_segs[2] = {{(void*)traces, j->in, in_bytes}, {(void*)reversed, j->rev, in_bytes}};
   if (banded) {
     // the checks and lazy set-up of sxc_plan_execute(_reversed), then the blocked run
@@ -1220,6 +1240,41 @@ void reference_call_omp(float* events, float* events_reversed, int n_events, int
   const size_t n = (size_t)np;
   std::vector<int> h(2 * (size_t)n_classes);
   const size_t in_bytes = (size_t)n_events * padded * sizeof(float);
+  HistArgs ha{};
+  ha.vpos = j->vp;
+  ha.lpos = j->lp;
+  ha.vneg = j->vn;
+  ha.lneg = j->ln;
+  ha.n = np;
+  ha.n_classes = n_classes;
+  ha.delta = (float)(1.0 / n_classes);
+  ha.threshold = threshold;
+  ha.hist_pos = j->hist;
+  ha.hist_neg = j->hist + n_classes;
+  if (n_events >= kBandMinEvents && blocking_enabled()) {
+    // blocked: inputs in and row bands out overlapped with compute, the
+    // histogram and threshold pass per row band
+    int rc = prepare_execute_reversed(j->p, j->in, j->rev, padded, padded);
+    if (!rc && hipMemset(j->hist, 0, 2 * (size_t)n_classes * sizeof(int)) != hipSuccess) {
+      set_error("clearing the histograms failed");
+      rc = SXC_ERR_HIP;
+    }
+    if (!rc) rc = run_blocked(j, events, events_reversed, padded, padded, vp, lp, vn, ln, &ha);
+    if (!rc && hipMemcpy(h.data(), j->hist, h.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("copying the histograms failed");
+      rc = SXC_ERR_HIP;
+    }
+    if (!rc) {
+      (void)check_tie_overflow(j->p);  // an overflow sets the error string; the results are delivered
+      for (int k = 0; k < n_classes; ++k) {
+        max_hist[k] += h[k];
+        min_hist[k] += h[n_classes + k];
+      }
+    }
+    if (rc == SXC_ERR_HIP) job_free(j);
+    else job_release(j);
+    return;
+  }
   bool ok = copy_to_device(j->in, events, in_bytes, nullptr) == SXC_OK &&
             copy_to_device(j->rev, events_reversed, in_bytes, nullptr) == SXC_OK &&
             hipMemset(j->hist, 0, 2 * (size_t)n_classes * sizeof(int)) == hipSuccess;
@@ -1228,17 +1283,6 @@ void reference_call_omp(float* events, float* events_reversed, int n_events, int
          SXC_OK;
   int tie_rc = SXC_OK;
   if (ok) {
-    HistArgs ha{};
-    ha.vpos = j->vp;
-    ha.lpos = j->lp;
-    ha.vneg = j->vn;
-    ha.lneg = j->ln;
-    ha.n = np;
-    ha.n_classes = n_classes;
-    ha.delta = (float)(1.0 / n_classes);
-    ha.threshold = threshold;
-    ha.hist_pos = j->hist;
-    ha.hist_neg = j->hist + n_classes;
     ok = launch_hist_threshold(ha, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     if (!ok) set_error("histogram pass failed");
     if (ok) tie_rc = check_tie_overflow(j->p);

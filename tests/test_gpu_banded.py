@@ -9,7 +9,7 @@ sxc_plan_execute_reversed."""
 import numpy as np
 import pytest
 
-from seismic_amd import correlation_lib, engine, synth
+from seismic_amd import correlation_lib, engine, omp_lib, synth
 from seismic_amd.device import DeviceArray, synchronize
 
 pytestmark = pytest.mark.gpu
@@ -62,4 +62,22 @@ def test_banded_host_call_demean_threshold():
     want, st = _device_execute(ev, P, lo, hi, L, threshold=thr, demean=True)
     assert st["near_ties"] > 0
     for name, g, w in zip(("vals_pos", "lags_pos", "vals_neg", "lags_neg"), got, want):
+        assert np.array_equal(g, w), name
+
+
+@pytest.mark.parametrize("thr,nc", [(0.2, 37), (0.0, 20000)])
+def test_banded_omp_abi_equals_unblocked(monkeypatch, thr, nc):
+    """The OMP ABI (correlationCPP): its histogram and threshold pass runs per
+    row band in the blocked run; values, lags and both histograms must equal
+    the unblocked run's (SEISMIC_XCORR_BLOCKED=0) bit for bit."""
+    n, L = 1100, 1024  # P = 2048, shift = L
+    ev, _ = synth.make_events(n, 1, L, seed=1101)
+    x = ev[:, 0]
+    print(f"case banded OMP ABI N={n} L={L} thr={thr} classes={nc}", flush=True)
+    monkeypatch.delenv("SEISMIC_XCORR_BLOCKED", raising=False)
+    got = omp_lib.compute_correlation(x, L, 8, 4, thr, nc)
+    monkeypatch.setenv("SEISMIC_XCORR_BLOCKED", "0")
+    want = omp_lib.compute_correlation(x, L, 8, 4, thr, nc)
+    assert int(got[4].sum()) == n * (n + 1) // 2
+    for name, g, w in zip(("vals_pos", "lags_pos", "vals_neg", "lags_neg", "hist_pos", "hist_neg"), got, want):
         assert np.array_equal(g, w), name
